@@ -1,0 +1,176 @@
+"""Benchmark: differentiable 3D Gaussian splatting, forward + backward per frame.
+
+Workload (one step = one frame): 1M seeded Gaussians (means uniform in a
+radius-2 ball, log-scales N(-4,0.5), random unit quaternions, opacity
+sigmoid(N(0,1)), colours U[0,1]), camera at distance 3, fov 60 deg, 1920x1080,
+16x16 tiles; loss = sum (rgb - target)^2 against a seeded U[0,1] image. A step
+runs projection, binning + stable sort, tiled raster, the loss gradient, the
+raster adjoint and the projection adjoint, all on one GPU, inputs resident in
+HBM. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "del-splat_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--num-gauss", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(s, m, shape, ts):
+    """The oracle (C++ restatement of del-splat-cpu) on one full frame,
+    single-threaded: projection, binning + stable sort, tiled raster, adjoint."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    from del_splat_dlpack import scene
+    tgt = scene.target_image(shape, 102)
+    t0 = time.perf_counter()
+    pxd, conic, aabb, _ = O.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
+    t2i, i2p = O.tile2pnt_gauss(pxd, aabb, shape, ts)
+    img, alpha, nwalk, tlast = O.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"], s["rgb"],
+                                                 ts, t2i, i2p)
+    d_img = (2.0 * (img - tgt)).astype(np.float32)
+    d = O.gauss_rasterize_bwd(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p,
+                              nwalk, tlast, d_img)
+    O.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape, d[0], d[1], log_scale=True)
+    dt = time.perf_counter() - t0
+    return {"value": round(dt * 1000.0, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+            "sample": f"one full frame ({s['xyz'].shape[0]} Gaussians, {shape[0]}x{shape[1]}, "
+                      "fwd+bwd) of the same workload, oracle C++ -O3, 1 thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import scene
+
+    dev = torch.device("cuda:0")
+    shape = (args.width, args.height)
+    ts = args.tile
+    s = scene.gaussians(args.num_gauss, 2, radius=2.0)
+    m = scene.front_camera(shape)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    m16 = t(m)
+    target = t(scene.target_image(shape, 102))
+
+    stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
+    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for k in stages}
+    acc = {k: 0.0 for k in stages}
+    kpairs = []
+
+    def step(timed):
+        r = {}
+        def mark(name, first):
+            if timed:
+                ev[name][0 if first else 1].record()
+        mark("project", True)
+        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
+        mark("project", False)
+        mark("bin", True)
+        t2i, i2p = SG.bin_gauss(shape, ts, pxd, aabb)
+        mark("bin", False)
+        mark("raster_fwd", True)
+        img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads)
+        mark("raster_fwd", False)
+        mark("loss_grad", True)
+        d_img = 2.0 * (img - target)
+        mark("loss_grad", False)
+        mark("raster_bwd", True)
+        dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast,
+                                                 d_img, None, args.threads)
+        mark("raster_bwd", False)
+        mark("project_bwd", True)
+        dxyz, dls, dq = SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
+        mark("project_bwd", False)
+        r["K"] = i2p.numel()
+        return r
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step(True)
+        kpairs.append(r["K"])
+        torch.cuda.synchronize()
+        for k in stages:
+            acc[k] += ev[k][0].elapsed_time(ev[k][1])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1000.0 / args.steps
+    stage_ms = {k: acc[k] / args.steps for k in stages}
+    K = kpairs[-1]
+    N = args.num_gauss
+    P = shape[0] * shape[1]
+    # algorithmic bytes per launch (implementation independent, reference layouts)
+    model = {
+        "raster_fwd": 40 * K + 24 * P,
+        "raster_bwd": 16 * P + 8 * P + 40 * K + 36 * N,
+        "project": 80 * N,
+        "project_bwd": 40 * N + 36 * N + 56 * N,
+    }
+    dom = max(("raster_fwd", "raster_bwd", "project", "project_bwd"), key=lambda k: stage_ms[k])
+    achieved = model[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(dom)
+        except Exception:
+            traffic = None
+    out = {
+        "metric": "fwd+bwd ms/frame @1M Gaussians 1080p",
+        "value": round(ms, 3),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussians and target image)",
+        "config": {"workload": f"3dgs fwd+bwd, {N} Gaussians, {shape[0]}x{shape[1]}, tile {ts}",
+                   "num_gauss": N, "width": shape[0], "height": shape[1], "tile": ts,
+                   "pairs_K": K, "parallelism": "single GPU"},
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s, m, shape, ts)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,89 @@
+// Shared device/host helpers of the dsplat HIP library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dsplat.h"
+
+#define DSPLAT_HIP(call)                                         \
+  do {                                                           \
+    hipError_t _e = (call);                                      \
+    if (_e != hipSuccess) return dsplat_set_hip_error(_e, #call); \
+  } while (0)
+
+// Records the HIP error text for dsplat_last_error() and returns
+// DSPLAT_ERR_HIP.
+int dsplat_set_hip_error(hipError_t e, const char* what);
+int dsplat_set_error(int code, const char* msg);
+
+namespace dsplat {
+
+constexpr int kWave = 64;
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+
+// Tile grid of an image (ceil, as splat_gauss.rs:59-62).
+inline void tile_grid(uint32_t w, uint32_t h, uint32_t ts, uint32_t* tw, uint32_t* th) {
+  *tw = (w + ts - 1) / ts;
+  *th = (h + ts - 1) / ts;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+__device__ inline uint32_t f32_bits(float v) { return __float_as_uint(v); }
+
+// Depth part of the binning key: bits of max(ndc_z + 1, 0)
+// (tile_acceleration.h:43-46). For non-negative floats the bit pattern
+// orders like the value.
+__device__ inline uint32_t depth_key(float ndcz) {
+  float zp1 = ndcz + 1.f;
+  if (zp1 <= 0.f) zp1 = 0.f;
+  return __float_as_uint(zp1);
+}
+
+// Half-open tile rectangle of a pixel box, clipped to the tile grid
+// (tile_acceleration.h:27-41). An inverted or NaN box covers no tile.
+struct TileRect {
+  int x0, x1, y0, y1;
+  __device__ int count() const { return (x1 - x0) * (y1 - y0); }
+};
+
+__device__ inline int tile_floor(float v, float tsf, uint32_t lim) {
+  float f = floorf(v / tsf);
+  f = fmaxf(f, -1.f);
+  f = fminf(f, float(lim));
+  return int(f);
+}
+
+__device__ inline TileRect tile_rect(float x0, float y0, float x1, float y1, uint32_t ts,
+                                     uint32_t tw, uint32_t th) {
+  TileRect r{0, 0, 0, 0};
+  if (!(x0 <= x1) || !(y0 <= y1)) return r;
+  const float tsf = float(ts);
+  r.x0 = max(tile_floor(x0, tsf, tw), 0);
+  r.x1 = min(tile_floor(x1, tsf, tw) + 1, int(tw));
+  r.y0 = max(tile_floor(y0, tsf, th), 0);
+  r.y1 = min(tile_floor(y1, tsf, th) + 1, int(th));
+  if (r.x1 < r.x0) r.x1 = r.x0;
+  if (r.y1 < r.y0) r.y1 = r.y0;
+  return r;
+}
+
+// Packed 2D splat record used by the raster kernels (one 64-byte line per
+// Gaussian): pixel centre, conic [a b c], opacity, rgb, pixel box.
+struct __align__(16) Splat2 {
+  float x, y, ca, cb;
+  float cc, op, r, g;
+  float b, x0, y0, x1;
+  float y1, pad0, pad1, pad2;
+};
+static_assert(sizeof(Splat2) == 64, "Splat2 must be one 64-byte line");
+
+}  // namespace dsplat

@@ -1,0 +1,338 @@
+// Device-wide primitives: exclusive scan (u32), stable LSD radix sort of
+// (key, u32 value) pairs, fill. They replace del-cudarc-sys's cumsum,
+// sort_by_key_u64 and array1d::fill used by the reference host layer
+// (del-splat-cudarc/src/splat_gauss.rs:137-146,184-191).
+//
+// Radix sort: 8-bit digits, one (upsweep, scan, scatter) triple per pass.
+// A 256-thread workgroup owns a 2048-element tile; each wave64 owns 512
+// consecutive elements and ranks them in rounds of 64 with a ballot-based
+// digit match, keeping per-wave digit counters in LDS, so the rank of an
+// element is (digit offset of earlier tiles) + (count in earlier waves of the
+// tile) + (count in earlier rounds of the wave) + (matching lanes below it).
+// That order is exactly element order: the sort is stable.
+#include "dsplat_common.hpp"
+#include "primitives.hpp"
+
+namespace dsplat {
+
+// ------------------------------------------------------------------ scan ----
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block; returns
+// the block total in *total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_inclusive_scan(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 64; ++k) {
+    const uint32_t s = wsum[k];
+    if (k < w) pre += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + inc - v;
+}
+
+__device__ inline uint32_t load_n(uint32_t n_cap, const uint32_t* d_n) {
+  if (!d_n) return n_cap;
+  const uint32_t v = *d_n;
+  return v < n_cap ? v : n_cap;
+}
+
+__global__ void __launch_bounds__(kScanBlock)
+scan_reduce_kernel(uint32_t n_cap, const uint32_t* d_n, const uint32_t* __restrict__ in,
+                   uint32_t* __restrict__ partial) {
+  const uint32_t n = load_n(n_cap, d_n);
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t i = base + uint64_t(k) * kScanBlock + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of nb partial sums, in place; also writes
+// the grand total to partial[nb].
+__global__ void __launch_bounds__(kScanBlock)
+scan_partials_kernel(uint32_t nb, uint32_t* __restrict__ partial, uint32_t* d_total) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kScanBlock) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? partial[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, &tot);
+    if (i < nb) partial[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    partial[nb] = carry;
+    if (d_total) *d_total = carry;
+  }
+}
+
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+__global__ void __launch_bounds__(kScanBlock)
+scan_down_kernel(uint32_t n_cap, const uint32_t* d_n, const uint32_t* __restrict__ in,
+                 uint32_t* __restrict__ out, const uint32_t* __restrict__ partial) {
+  const uint32_t n = load_n(n_cap, d_n);
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+}
+
+size_t scan_workspace_bytes(uint32_t n) {
+  const uint32_t nb = div_up(n, kScanTile) + 1;
+  return (size_t(nb) + 1) * sizeof(uint32_t) + 256;
+}
+
+int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uint32_t* in,
+                   uint32_t* out, uint32_t* ws, uint32_t* d_total) {
+  if (n_cap == 0) {
+    if (d_total) set_u32_kernel<<<1, 1, 0, s>>>(d_total, 0u);
+    DSPLAT_HIP(hipGetLastError());
+    return DSPLAT_OK;
+  }
+  const uint32_t nb = div_up(n_cap, kScanTile);
+  scan_reduce_kernel<<<nb, kScanBlock, 0, s>>>(n_cap, d_n, in, ws);
+  scan_partials_kernel<<<1, kScanBlock, 0, s>>>(nb, ws, d_total);
+  scan_down_kernel<<<nb, kScanBlock, 0, s>>>(n_cap, d_n, in, out, ws);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+// ------------------------------------------------------------ radix sort ----
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;                        // rounds per wave
+constexpr int kSortWaveChunk = 64 * kSortItems;      // 512
+constexpr int kSortTile = kSortBlock * kSortItems;   // 2048
+constexpr int kSortWaves = kSortBlock / 64;
+
+template <typename KeyT>
+__global__ void __launch_bounds__(kSortBlock)
+radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ keys,
+                     uint32_t shift, uint32_t nbins, uint32_t nblocks,
+                     uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  const uint32_t n = load_n(n_cap, d_n);
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+  const uint32_t mask = nbins - 1;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < nbins) hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <typename KeyT>
+__global__ void __launch_bounds__(kSortBlock)
+radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
+                     const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
+                     uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
+                     uint32_t nblocks, const uint32_t* __restrict__ hist) {
+  __shared__ uint32_t wcnt[kSortWaves][256];
+  const uint32_t n = load_n(n_cap, d_n);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
+  __syncthreads();
+  const uint32_t mask = (1u << dbits) - 1u;
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile + uint64_t(w) * kSortWaveChunk;
+  KeyT key[kSortItems];
+  uint32_t val[kSortItems], rank[kSortItems], dig[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * 64 + lane;
+    const bool valid = i < n;
+    key[r] = valid ? kin[i] : KeyT(0);
+    val[r] = valid ? vin[i] : 0u;
+    const uint32_t d = uint32_t(key[r] >> shift) & mask;
+    dig[r] = valid ? d : 0xffffffffu;
+    uint64_t peers = __ballot(valid);
+    for (uint32_t b = 0; b < dbits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t below = lanes_below(peers);
+    uint32_t running = 0;
+    if (valid) running = wcnt[w][d];
+    rank[r] = running + below;
+    if (valid && below == 0) wcnt[w][d] = running + uint32_t(__popcll(peers));
+  }
+  __syncthreads();
+  // exclusive prefix of the per-wave counts over waves, per digit
+  if (threadIdx.x < 256) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kSortWaves; ++k) {
+      const uint32_t c = wcnt[k][threadIdx.x];
+      wcnt[k][threadIdx.x] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (dig[r] == 0xffffffffu) continue;
+    const uint32_t d = dig[r];
+    const uint32_t pos = hist[d * nblocks + blockIdx.x] + wcnt[w][d] + rank[r];
+    kout[pos] = key[r];
+    vout[pos] = val[r];
+  }
+}
+
+template <typename KeyT>
+size_t sort_workspace_bytes(uint32_t n) {
+  const uint32_t nb = div_up(n ? n : 1, kSortTile);
+  size_t b = 0;
+  b += (size_t(n) * sizeof(KeyT) + 255) & ~size_t(255);         // alternate keys
+  b += (size_t(n) * sizeof(uint32_t) + 255) & ~size_t(255);     // alternate values
+  b += (size_t(256) * nb * sizeof(uint32_t) + 255) & ~size_t(255);  // histograms
+  b += scan_workspace_bytes(256u * nb);
+  return b;
+}
+
+template <typename KeyT>
+int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
+                     uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
+                     bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out) {
+  *result_in_alt = false;
+  if (ws_bytes < sort_workspace_bytes<KeyT>(n_cap))
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
+  if (key_bits == 0 || key_bits > 8 * sizeof(KeyT))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "radix sort: key_bits out of range");
+  const uint32_t nb = div_up(n_cap ? n_cap : 1, kSortTile);
+  char* p = static_cast<char*>(ws);
+  KeyT* kalt = reinterpret_cast<KeyT*>(p);
+  p += (size_t(n_cap) * sizeof(KeyT) + 255) & ~size_t(255);
+  uint32_t* valt = reinterpret_cast<uint32_t*>(p);
+  p += (size_t(n_cap) * sizeof(uint32_t) + 255) & ~size_t(255);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(p);
+  p += (size_t(256) * nb * sizeof(uint32_t) + 255) & ~size_t(255);
+  uint32_t* scan_ws = reinterpret_cast<uint32_t*>(p);
+  if (keys_alt_out) *keys_alt_out = kalt;
+  if (vals_alt_out) *vals_alt_out = valt;
+  if (n_cap == 0) return DSPLAT_OK;
+  KeyT* ksrc = keys; uint32_t* vsrc = vals; KeyT* kdst = kalt; uint32_t* vdst = valt;
+  bool in_alt = false;
+  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    const uint32_t dbits = key_bits - shift < 8 ? key_bits - shift : 8;
+    const uint32_t nbins = 1u << dbits;
+    radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, nbins, nb, hist);
+    int rc = exclusive_scan(s, nbins * nb, nullptr, hist, hist, scan_ws, nullptr);
+    if (rc) return rc;
+    radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, vsrc, kdst, vdst,
+                                                         shift, dbits, nb, hist);
+    std::swap(ksrc, kdst);
+    std::swap(vsrc, vdst);
+    in_alt = !in_alt;
+  }
+  DSPLAT_HIP(hipGetLastError());
+  *result_in_alt = in_alt;
+  return DSPLAT_OK;
+}
+
+template size_t sort_workspace_bytes<uint32_t>(uint32_t);
+template size_t sort_workspace_bytes<uint64_t>(uint32_t);
+template int radix_sort_pairs<uint32_t>(hipStream_t, uint32_t, const uint32_t*, uint32_t*,
+                                        uint32_t*, uint32_t, void*, size_t, bool*, uint32_t**,
+                                        uint32_t**);
+template int radix_sort_pairs<uint64_t>(hipStream_t, uint32_t, const uint32_t*, uint64_t*,
+                                        uint32_t*, uint32_t, void*, size_t, bool*, uint64_t**,
+                                        uint32_t**);
+
+// ------------------------------------------------------------------ fill ----
+__global__ void fill_f32_kernel(uint32_t n, float* __restrict__ buf, float v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) buf[i] = v;
+}
+
+int fill_f32(hipStream_t s, uint32_t n, float* buf, float v) {
+  if (n == 0) return DSPLAT_OK;
+  fill_f32_kernel<<<div_up(n, 256), 256, 0, s>>>(n, buf, v);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+}  // namespace dsplat
+
+// ------------------------------------------------------------------ C ABI ---
+using namespace dsplat;
+
+extern "C" size_t dsplat_exclusive_scan_workspace_size(uint32_t n) {
+  return scan_workspace_bytes(n);
+}
+
+extern "C" int dsplat_exclusive_scan_u32(void* stream, uint32_t n, const uint32_t* in,
+                                         uint32_t* out, void* workspace,
+                                         size_t workspace_bytes) {
+  if (n && (!in || !out)) return dsplat_set_error(DSPLAT_ERR_ARG, "scan: null buffer");
+  if (workspace_bytes < scan_workspace_bytes(n) || (n && !workspace))
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "scan: workspace too small");
+  return exclusive_scan(as_stream(stream), n, nullptr, in, out,
+                        static_cast<uint32_t*>(workspace), nullptr);
+}
+
+extern "C" size_t dsplat_radix_sort_workspace_size(uint32_t n) {
+  return sort_workspace_bytes<uint64_t>(n);
+}
+
+extern "C" int dsplat_radix_sort_by_key_u64_u32(void* stream, uint32_t n, uint64_t* keys,
+                                                uint32_t* vals, uint32_t key_bits,
+                                                void* workspace, size_t workspace_bytes) {
+  if (n && (!keys || !vals)) return dsplat_set_error(DSPLAT_ERR_ARG, "sort: null buffer");
+  hipStream_t s = as_stream(stream);
+  bool in_alt = false;
+  uint64_t* kalt = nullptr;
+  uint32_t* valt = nullptr;
+  int rc = radix_sort_pairs<uint64_t>(s, n, nullptr, keys, vals, key_bits, workspace,
+                                      workspace_bytes, &in_alt, &kalt, &valt);
+  if (rc) return rc;
+  if (in_alt && n) {
+    DSPLAT_HIP(hipMemcpyAsync(keys, kalt, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    DSPLAT_HIP(hipMemcpyAsync(vals, valt, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+  }
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_fill_f32(void* stream, uint32_t n, float* buf, float value) {
+  if (n && !buf) return dsplat_set_error(DSPLAT_ERR_ARG, "fill: null buffer");
+  return fill_f32(as_stream(stream), n, buf, value);
+}

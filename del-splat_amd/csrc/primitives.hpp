@@ -1,0 +1,31 @@
+// Internal interface of the device-wide primitives (primitives.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace dsplat {
+
+// Workspace bytes of exclusive_scan for n elements.
+size_t scan_workspace_bytes(uint32_t n);
+
+// out[i] = sum_{j<i} in[j] over n = min(*d_n, n_cap) elements (d_n may be
+// null: n = n_cap). in == out allowed. If d_total != null the total is
+// written there. ws: scan_workspace_bytes(n_cap) bytes.
+int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uint32_t* in,
+                   uint32_t* out, uint32_t* ws, uint32_t* d_total);
+
+template <typename KeyT>
+size_t sort_workspace_bytes(uint32_t n);
+
+// Stable LSD radix sort of (keys, vals) over the low key_bits bits, n =
+// min(*d_n, n_cap). The result is in (keys, vals) if *result_in_alt is false,
+// else in the workspace's alternate buffers (*keys_alt, *vals_alt).
+template <typename KeyT>
+int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
+                     uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
+                     bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt);
+
+int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
+
+}  // namespace dsplat

@@ -1,0 +1,202 @@
+"""Gaussian splatting on MI355X, torch front end.
+
+The stage functions mirror del-splat-cudarc's Gaussian host API
+(del-splat-cudarc/src/splat_gauss.rs): `pnt2splat3_to_pnt2splat2` (:6-44),
+`tile2idx_idx2pnt` (:96-195), `rasterize_pnt2splat2` (:46-94). `render` is
+the differentiable renderer (no counterpart in the reference): a
+torch.autograd.Function whose backward is the hand-written adjoint kernels.
+All tensors live on the GPU; torch only provides memory and the stream.
+"""
+import torch
+
+from .. import _native
+from .._native import call
+from .._torch_util import (check_f32, ptr, require_gpu, stream_ptr, tile_shape,
+                           world2ndc_col_major)
+
+DEFAULT_TILE = 16
+
+
+def pnt2splat3_to_pnt2splat2(pnt2xyz, pnt2scale, pnt2quat, transform_world2ndc, img_shape):
+    """Projection (splat_gauss.rs:6-44): returns (pnt2pixxydepth [N,3],
+    pnt2pixconvinv [N,3], pnt2pixaabb [N,4]). pnt2scale holds sigma."""
+    require_gpu(pnt2xyz, "pnt2xyz")
+    dev = pnt2xyz.device
+    n = pnt2xyz.shape[0]
+    check_f32(pnt2xyz, (n, 3), "pnt2xyz")
+    check_f32(pnt2scale, (n, 3), "pnt2scale")
+    check_f32(pnt2quat, (n, 4), "pnt2quat")
+    m = world2ndc_col_major(transform_world2ndc, dev)
+    pxd = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    conic = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    aabb = torch.empty((n, 4), device=dev, dtype=torch.float32)
+    call("dsplat_gauss_pnt2splat3_to_pnt2splat2", stream_ptr(dev), n,
+         ptr(pnt2xyz.contiguous()), ptr(pnt2scale.contiguous()), ptr(pnt2quat.contiguous()),
+         ptr(pxd), ptr(conic), ptr(aabb), ptr(m), img_shape[0], img_shape[1])
+    return pxd, conic, aabb
+
+
+def tile2idx_idx2pnt(img_shape, tile_size, pnt2pixxydepth, pnt2pixaabb):
+    """Binning + stable sort (splat_gauss.rs:96-195): returns (tile2idx
+    u32[T+1], idx2pnt u32[K]) as int32-typed tensors holding u32 values.
+    Reads K back with one blocking copy, as the reference does."""
+    require_gpu(pnt2pixaabb, "pnt2pixaabb")
+    dev = pnt2pixaabb.device
+    n = pnt2pixaabb.shape[0]
+    tw, th = tile_shape(img_shape, tile_size)
+    s = stream_ptr(dev)
+    kbuf = torch.zeros(4, device=dev, dtype=torch.int32)
+    pxd = pnt2pixxydepth.contiguous()
+    bb = pnt2pixaabb.contiguous()
+    call("dsplat_gauss_count_pairs", s, img_shape[0], img_shape[1], tile_size, n, ptr(bb),
+         ptr(kbuf))
+    k = int(kbuf[0].item())
+    return _bin_with_capacity(img_shape, tile_size, n, pxd, bb, k, kbuf)
+
+
+def _bin_with_capacity(img_shape, tile_size, n, pxd, bb, cap, kbuf):
+    dev = pxd.device
+    tw, th = tile_shape(img_shape, tile_size)
+    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, cap)
+    ws = torch.empty(ws_bytes, device=dev, dtype=torch.uint8)
+    tile2idx = torch.empty(tw * th + 1, device=dev, dtype=torch.int32)
+    idx2pnt = torch.empty(max(cap, 1), device=dev, dtype=torch.int32)
+    call("dsplat_gauss_tile2idx_idx2pnt_ws", stream_ptr(dev), img_shape[0], img_shape[1],
+         tile_size, n, ptr(pxd), ptr(bb), ptr(tile2idx), ptr(idx2pnt), cap, ptr(ws), ws_bytes,
+         ptr(kbuf))
+    return tile2idx, idx2pnt[:cap]
+
+
+def rasterize_pnt2splat2(img_shape, pnt2pixxydepth, pnt2pixconvinv, pnt2pixaabb, pnt2opacity,
+                         pnt2rgb, tile_size, tile2idx, idx2pnt):
+    """Tiled front-to-back raster (splat_gauss.rs:46-94) -> pix2rgb [H,W,3]."""
+    require_gpu(pnt2pixaabb, "pnt2pixaabb")
+    dev = pnt2pixaabb.device
+    w, h = img_shape
+    img = torch.empty((h, w, 3), device=dev, dtype=torch.float32)
+    i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
+    call("dsplat_gauss_rasterize_pnt2splat2", stream_ptr(dev), w, h, ptr(img),
+         ptr(pnt2pixxydepth.contiguous()), ptr(pnt2pixconvinv.contiguous()),
+         ptr(pnt2pixaabb.contiguous()), ptr(pnt2opacity.contiguous()), ptr(pnt2rgb.contiguous()),
+         tile_size, ptr(tile2idx), ptr(i2p))
+    return img
+
+
+# ------------------------------------------------------------------ render --
+def project_fwd(xyz, scale, quat, opacity, rgb, m16, img_shape, log_scale=True):
+    """Projection with the packed raster record. Returns (pixxydepth, conic,
+    aabb, splat2 [N,16])."""
+    dev = xyz.device
+    n = xyz.shape[0]
+    pxd = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    conic = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    aabb = torch.empty((n, 4), device=dev, dtype=torch.float32)
+    splat2 = torch.empty((n, 16), device=dev, dtype=torch.float32)
+    call("dsplat_gauss_project_fwd", stream_ptr(dev), n, ptr(xyz), ptr(scale), ptr(quat),
+         ptr(opacity), ptr(rgb), ptr(m16), img_shape[0], img_shape[1],
+         _native.DSPLAT_SCALE_IS_LOG if log_scale else 0, ptr(pxd), ptr(conic), ptr(aabb),
+         ptr(splat2))
+    return pxd, conic, aabb, splat2
+
+
+def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0):
+    """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tlast [H,W])."""
+    dev = splat2.device
+    w, h = img_shape
+    img = torch.empty((h, w, 3), device=dev, dtype=torch.float32)
+    alpha = torch.empty((h, w), device=dev, dtype=torch.float32)
+    nwalk = torch.empty((h, w), device=dev, dtype=torch.int32)
+    tlast = torch.empty((h, w), device=dev, dtype=torch.float32)
+    i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
+    call("dsplat_gauss_rasterize_fwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
+         ptr(tile2idx), ptr(i2p), ptr(img), ptr(alpha), ptr(nwalk), ptr(tlast))
+    return img, alpha, nwalk, tlast
+
+
+def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, d_img,
+                  d_alpha=None, threads=0):
+    """Returns 2D gradients (d_pixxy [N,2], d_conic [N,3], d_opacity [N],
+    d_rgb [N,3])."""
+    dev = splat2.device
+    n = splat2.shape[0]
+    w, h = img_shape
+    d_pixxy = torch.zeros((n, 2), device=dev, dtype=torch.float32)
+    d_conic = torch.zeros((n, 3), device=dev, dtype=torch.float32)
+    d_op = torch.zeros(n, device=dev, dtype=torch.float32)
+    d_rgb = torch.zeros((n, 3), device=dev, dtype=torch.float32)
+    i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
+    da = None if d_alpha is None else d_alpha.contiguous()
+    call("dsplat_gauss_rasterize_bwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
+         ptr(tile2idx), ptr(i2p), ptr(nwalk), ptr(tlast), ptr(d_img.contiguous()), ptr(da),
+         ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
+    return d_pixxy, d_conic, d_op, d_rgb
+
+
+def project_bwd(xyz, scale, quat, m16, img_shape, d_pixxy, d_conic, log_scale=True):
+    """Returns (d_xyz [N,3], d_scale [N,3], d_quat [N,4])."""
+    dev = xyz.device
+    n = xyz.shape[0]
+    d_xyz = torch.zeros((n, 3), device=dev, dtype=torch.float32)
+    d_scale = torch.zeros((n, 3), device=dev, dtype=torch.float32)
+    d_quat = torch.zeros((n, 4), device=dev, dtype=torch.float32)
+    call("dsplat_gauss_project_bwd", stream_ptr(dev), n, ptr(xyz), ptr(scale), ptr(quat),
+         ptr(m16), img_shape[0], img_shape[1], _native.DSPLAT_SCALE_IS_LOG if log_scale else 0,
+         ptr(d_pixxy), ptr(d_conic), ptr(d_xyz), ptr(d_scale), ptr(d_quat))
+    return d_xyz, d_scale, d_quat
+
+
+def bin_gauss(img_shape, tile_size, pxd, aabb):
+    return tile2idx_idx2pnt(img_shape, tile_size, pxd, aabb)
+
+
+class _Render(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, log_scale, quat, opacity, rgb, m16, img_shape, tile_size, threads):
+        xyz, log_scale, quat, opacity, rgb = (t.contiguous() for t in
+                                              (xyz, log_scale, quat, opacity, rgb))
+        pxd, conic, aabb, splat2 = project_fwd(xyz, log_scale, quat, opacity, rgb, m16,
+                                               img_shape, log_scale=True)
+        tile2idx, idx2pnt = bin_gauss(img_shape, tile_size, pxd, aabb)
+        img, alpha, nwalk, tlast = rasterize_fwd(img_shape, tile_size, splat2, tile2idx,
+                                                 idx2pnt, threads)
+        ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast)
+        ctx.img_shape = img_shape
+        ctx.tile_size = tile_size
+        ctx.threads = threads
+        ctx.mark_non_differentiable(nwalk)
+        return img, alpha
+
+    @staticmethod
+    def backward(ctx, d_img, d_alpha):
+        xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast = ctx.saved_tensors
+        if d_img is None:
+            d_img = torch.zeros((ctx.img_shape[1], ctx.img_shape[0], 3), device=xyz.device)
+        d_pixxy, d_conic, d_op, d_rgb = rasterize_bwd(ctx.img_shape, ctx.tile_size, splat2,
+                                                      tile2idx, idx2pnt, nwalk, tlast, d_img,
+                                                      d_alpha, ctx.threads)
+        d_xyz, d_ls, d_quat = project_bwd(xyz, log_scale, quat, m16, ctx.img_shape, d_pixxy,
+                                          d_conic, log_scale=True)
+        return d_xyz, d_ls, d_quat, d_op, d_rgb, None, None, None, None
+
+
+def render(xyz, log_scale, quat, opacity, rgb, transform_world2ndc, img_shape,
+           tile_size=DEFAULT_TILE, threads=0):
+    """Differentiable render of N Gaussians -> (rgb [H,W,3], alpha [H,W]).
+
+    xyz [N,3] means, log_scale [N,3] log sigma, quat [N,4] (x,y,z,w; used as
+    given), opacity [N] in (0,1), rgb [N,3]; transform_world2ndc (4,4)
+    row-major or (16,) column-major; img_shape (W, H). Gradients flow to all
+    five per-Gaussian inputs through the adjoint kernels.
+    """
+    require_gpu(xyz, "xyz")
+    n = xyz.shape[0]
+    check_f32(xyz, (n, 3), "xyz")
+    check_f32(log_scale, (n, 3), "log_scale")
+    check_f32(quat, (n, 4), "quat")
+    check_f32(opacity, (n,), "opacity")
+    check_f32(rgb, (n, 3), "rgb")
+    if tile_size not in (8, 16, 32):
+        raise ValueError("tile_size must be 8, 16 or 32")
+    m16 = world2ndc_col_major(transform_world2ndc, xyz.device)
+    return _Render.apply(xyz, log_scale, quat, opacity, rgb, m16, tuple(img_shape), tile_size,
+                         threads)

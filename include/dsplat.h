@@ -1,0 +1,263 @@
+/*
+ * dsplat -- C ABI of the MI355X (gfx950) splatting library.
+ *
+ * Drop-in boundary for the del-splat hot path. Each entry point replaces one
+ * function of the reference's host layer (del-splat-cudarc, Rust over the
+ * CUDA driver API) or one primitive of del-cudarc-sys that layer calls; the
+ * comment above each declaration cites the reference interface (file:line,
+ * relative to the del-splat repository).
+ *
+ * Conventions
+ *  - `stream` is a hipStream_t passed as void* (0 = the null stream). All work
+ *    is enqueued on it asynchronously; only the functions marked SYNC block.
+ *  - Every pointer argument is device memory unless marked "host".
+ *  - Arrays are flat, row-major, float32 unless the type says otherwise:
+ *    pnt2xyz [N,3], pnt2quat [N,4] (x,y,z,w), pnt2scale [N,3],
+ *    pnt2pixxydepth [N,3] (pixel x, pixel y, ndc z), pnt2pixconvinv [N,3]
+ *    (conic a,b,c of [[a,b],[b,c]]), pnt2pixaabb [N,4] (x0,y0,x1,y1),
+ *    images [H,W,C] with pixel index iy*W+ix.
+ *  - transform_world2ndc is the column-major 4x4 (m[i+4j] = M_ij), 16 floats
+ *    in device memory, as in the reference.
+ *  - Tile grid: tile_w = ceil(W/ts), tile_h = ceil(H/ts), T = tile_w*tile_h,
+ *    tile index = iy_tile*tile_w + ix_tile; tile_size ts in {8, 16, 32}.
+ *  - Return value: DSPLAT_OK (0) or a negative DSPLAT_ERR_*; the library never
+ *    aborts. dsplat_last_error() returns the text of the last error of the
+ *    calling thread.
+ */
+#ifndef DSPLAT_H_
+#define DSPLAT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSPLAT_OK 0
+#define DSPLAT_ERR_ARG (-1)       /* invalid argument (size, tile size, null) */
+#define DSPLAT_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define DSPLAT_ERR_CAPACITY (-3)  /* workspace or capacity too small */
+
+/* Text of the calling thread's last error ("" if none). */
+const char* dsplat_last_error(void);
+
+/* Library version, as major*10000 + minor*100 + patch. */
+int dsplat_version(void);
+
+/* ======================================================================
+ * Gaussian path (del-splat-cudarc/src/splat_gauss.rs)
+ * ====================================================================== */
+
+/* 3D -> 2D projection of N Gaussians.
+ * Replaces splat_gauss::pnt2splat3_to_pnt2splat2 (splat_gauss.rs:6-44,
+ * kernel splat_gauss.cu:10-53). pnt2scale holds the standard deviations
+ * sigma (covariance R diag(sigma^2) R^T). Writes pnt2pixxydepth,
+ * pnt2pixconvinv (conic), pnt2pixaabb (3-sigma pixel box). */
+int dsplat_gauss_pnt2splat3_to_pnt2splat2(void* stream, uint32_t num_pnt, const float* pnt2xyz,
+                                          const float* pnt2scale, const float* pnt2quat,
+                                          float* pnt2pixxydepth, float* pnt2pixconvinv,
+                                          float* pnt2pixaabb, const float* transform_world2ndc,
+                                          uint32_t img_w, uint32_t img_h);
+
+/* Number of (tile, Gaussian) pairs K for the given boxes, written to the
+ * device u32 *d_num_idx. First half of splat_gauss::tile2idx_idx2pnt
+ * (splat_gauss.rs:104-147: count + scan, whose total the reference reads back
+ * with a blocking copy). */
+int dsplat_gauss_count_pairs(void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size,
+                             uint32_t num_pnt, const float* pnt2pixaabb, uint32_t* d_num_idx);
+
+/* Workspace bytes needed by dsplat_gauss_tile2idx_idx2pnt_ws /
+ * dsplat_circle_tile2idx_idx2pnt_ws for N Gaussians, T tiles and a pair
+ * capacity. Host function, no device work. */
+size_t dsplat_tile2idx_idx2pnt_workspace_size(uint32_t num_pnt, uint32_t num_tile,
+                                              uint32_t capacity);
+
+/* Binning + stable sort without host synchronisation (graph-capturable).
+ * Replaces splat_gauss::tile2idx_idx2pnt (splat_gauss.rs:96-195) with caller-
+ * owned outputs: tile2idx u32[T+1] (tile t owns idx2pnt[tile2idx[t] ..
+ * tile2idx[t+1])), idx2pnt u32[capacity]. Within a tile the Gaussians are
+ * ordered by ascending bits(max(ndc_z+1,0)), ties by ascending index -- the
+ * order of the reference's stable sort of (tile<<32 | depth bits) keys.
+ * K is written to the device u32 *d_num_idx. If K > capacity nothing else is
+ * written: read *d_num_idx and call again with a larger capacity. */
+int dsplat_gauss_tile2idx_idx2pnt_ws(void* stream, uint32_t img_w, uint32_t img_h,
+                                     uint32_t tile_size, uint32_t num_pnt,
+                                     const float* pnt2pixxydepth, const float* pnt2pixaabb,
+                                     uint32_t* tile2idx, uint32_t* idx2pnt, uint32_t capacity,
+                                     void* workspace, size_t workspace_bytes,
+                                     uint32_t* d_num_idx);
+
+/* SYNC. Same result as the _ws variant with the reference's ownership: the
+ * library allocates idx2pnt (exactly K entries, returned in *idx2pnt and the
+ * host *num_idx; release it with dsplat_free) after a blocking read of K, as
+ * splat_gauss.rs:147-159 does. tile2idx u32[T+1] is caller-owned. */
+int dsplat_gauss_tile2idx_idx2pnt(void* stream, uint32_t img_w, uint32_t img_h,
+                                  uint32_t tile_size, uint32_t num_pnt,
+                                  const float* pnt2pixxydepth, const float* pnt2pixaabb,
+                                  uint32_t* tile2idx, uint32_t** idx2pnt, uint32_t* num_idx);
+
+/* Releases memory returned by the library (stream-ordered on `stream`). */
+int dsplat_free(void* stream, void* ptr);
+
+/* Tiled front-to-back alpha blending into pix2rgb [H,W,3] (overwritten; the
+ * reference accumulates into an uninitialised buffer).
+ * Replaces splat_gauss::rasterize_pnt2splat2 (splat_gauss.rs:46-94, kernel
+ * splat_gauss.cu:56-109). num_tile+1 entries of tile2idx are read. */
+int dsplat_gauss_rasterize_pnt2splat2(void* stream, uint32_t img_w, uint32_t img_h,
+                                      float* pix2rgb, const float* pnt2pixxydepth,
+                                      const float* pnt2pixconvinv, const float* pnt2pixaabb,
+                                      const float* pnt2opacity, const float* pnt2rgb,
+                                      uint32_t tile_size, const uint32_t* tile2idx,
+                                      const uint32_t* idx2pnt);
+
+/* ---- differentiable path (no counterpart in the reference) ---- */
+
+#define DSPLAT_SCALE_IS_LOG 1u  /* pnt2scale holds log(sigma) */
+
+/* Projection for the differentiable path: as
+ * dsplat_gauss_pnt2splat3_to_pnt2splat2 (flags may select log-scales) and in
+ * addition the packed per-Gaussian raster record pnt2splat2 [N,16] float
+ * (x, y, a, b, c, opacity, r, g, b, x0, y0, x1, y1, 0, 0, 0). */
+int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,
+                             const float* pnt2scale, const float* pnt2quat,
+                             const float* pnt2opacity, const float* pnt2rgb,
+                             const float* transform_world2ndc, uint32_t img_w, uint32_t img_h,
+                             uint32_t flags, float* pnt2pixxydepth, float* pnt2pixconvinv,
+                             float* pnt2pixaabb, float* pnt2splat2);
+
+/* Tiled raster over packed records, with the state the adjoint needs:
+ * pix2rgb [H,W,3], pix2alpha [H,W] (the reference's alpha_sum), pix2nwalk
+ * u32 [H,W] (tile-list entries walked, including the one that terminated),
+ * pix2tlast [H,W] (transmittance in front of the last contributing splat).
+ * pix2alpha / pix2nwalk / pix2tlast may be null. `threads` selects the
+ * workgroup size per tile (0 = default; ts 16: 64/128/256, ts 32:
+ * 256/512/1024; ts 8: always 64). */
+int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
+                               uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
+                               const uint32_t* tile2idx, const uint32_t* idx2pnt,
+                               float* pix2rgb, float* pix2alpha, uint32_t* pix2nwalk,
+                               float* pix2tlast);
+
+/* Adjoint of dsplat_gauss_rasterize_fwd. Walks every pixel's list
+ * back-to-front from pix2tlast. ADDS into d_pnt2pixxy [N,2], d_pnt2conic
+ * [N,3], d_pnt2opacity [N], d_pnt2rgb [N,3]. d_pix2alpha may be null.
+ * `threads` as for dsplat_gauss_rasterize_fwd (ts 32: 256 only). */
+int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
+                               uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
+                               const uint32_t* tile2idx, const uint32_t* idx2pnt,
+                               const uint32_t* pix2nwalk, const float* pix2tlast,
+                               const float* d_pix2rgb, const float* d_pix2alpha,
+                               float* d_pnt2pixxy, float* d_pnt2conic, float* d_pnt2opacity,
+                               float* d_pnt2rgb);
+
+/* Adjoint of the projection. ADDS into d_pnt2xyz [N,3], d_pnt2scale [N,3]
+ * (w.r.t. sigma, or log sigma under DSPLAT_SCALE_IS_LOG), d_pnt2quat [N,4]. */
+int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,
+                             const float* pnt2scale, const float* pnt2quat,
+                             const float* transform_world2ndc, uint32_t img_w, uint32_t img_h,
+                             uint32_t flags, const float* d_pnt2pixxy, const float* d_pnt2conic,
+                             float* d_pnt2xyz, float* d_pnt2scale, float* d_pnt2quat);
+
+/* ======================================================================
+ * Circle path (del-splat-cudarc/src/splat_circle.rs) and point z-buffers
+ * (del-splat-dlpack/src/splat_point.rs)
+ * ====================================================================== */
+
+/* Sphere -> screen circle: pnt2pixxydepth [N,3], pnt2pixrad [N] (pixels).
+ * Replaces splat_circle::pnt2splat3_to_pnt2splat2 (splat_circle.rs:96-132,
+ * kernel splat_circle.cu:7-41) and the GPU branch of
+ * circle_splat_from_project_spheres (del-splat-dlpack/src/splat_circle.rs:
+ * 51-71, whose module lookup is broken in the reference). */
+int dsplat_circle_pnt2splat3_to_pnt2splat2(void* stream, uint32_t num_pnt, const float* pnt2xyz,
+                                           float* pnt2pixxydepth, float* pnt2pixrad,
+                                           const float* transform_world2ndc, uint32_t img_w,
+                                           uint32_t img_h, float radius);
+
+/* Circle binning, graph-capturable. Replaces splat_circle::tile2idx_idx2pnt
+ * (splat_circle.rs:3-94; it takes the tile shape, and the tile size the
+ * reference hard-codes to 16 is an argument here). Boxes are
+ * [x-r, y-r, x+r, y+r]; otherwise as dsplat_gauss_tile2idx_idx2pnt_ws. */
+int dsplat_circle_tile2idx_idx2pnt_ws(void* stream, uint32_t tile_w, uint32_t tile_h,
+                                      uint32_t tile_size, uint32_t num_pnt,
+                                      const float* pnt2pixxydepth, const float* pnt2pixrad,
+                                      uint32_t* tile2idx, uint32_t* idx2pnt, uint32_t capacity,
+                                      void* workspace, size_t workspace_bytes,
+                                      uint32_t* d_num_idx);
+
+/* SYNC. Allocating variant (release idx2pnt with dsplat_free). */
+int dsplat_circle_tile2idx_idx2pnt(void* stream, uint32_t tile_w, uint32_t tile_h,
+                                   uint32_t tile_size, uint32_t num_pnt,
+                                   const float* pnt2pixxydepth, const float* pnt2pixrad,
+                                   uint32_t* tile2idx, uint32_t** idx2pnt, uint32_t* num_idx);
+
+/* Tiled painter: per pixel the last list entry (ascending key order) whose
+ * circle covers the pixel centre gives the colour; uncovered pixels are 0.
+ * Replaces splat_circle::splat (splat_circle.rs:134-177, kernel
+ * splat_circle.cu:93-132). */
+int dsplat_circle_splat(void* stream, uint32_t img_w, uint32_t img_h, float* pix2rgb,
+                        const float* pnt2pixxydepth, const float* pnt2pixrad,
+                        const float* pnt2rgb, uint32_t tile_size, const uint32_t* tile2idx,
+                        const uint32_t* idx2pnt);
+
+/* Deterministic point z-buffer. Replaces point_splat_rasterize_pix_zbuffer's
+ * GPU branch (del-splat-dlpack/src/splat_point.rs:80-103, kernel
+ * splat_point.cu:49-74): the pixel floor(x), floor(y) of every point with
+ * ndc_z in [-1,1] keeps min(bits(1-(ndc_z+1)/2) << 32 | index) in pix2key
+ * u64 [H,W] (caller-owned scratch); then pix2unitdepth [H,W] gets the winning
+ * depth (FLT_MAX where empty, as the reference's fill) and pix2rgb [H,W,3]
+ * its colour (0 where empty). pix2unitdepth may be null. */
+int dsplat_point_rasterize_zbuffer(void* stream, uint32_t num_pnt, const float* pnt2pixxyndcz,
+                                   const float* pnt2rgb, float* pix2unitdepth, float* pix2rgb,
+                                   uint32_t img_w, uint32_t img_h, uint64_t* pix2key);
+
+/* As dsplat_point_rasterize_zbuffer, but every point covers all pixels whose
+ * centre is within pnt2pixrad [N] of it (the circle test of
+ * splat_circle.cu:120-125). No counterpart in the reference. */
+int dsplat_circle_rasterize_zbuffer(void* stream, uint32_t num_pnt, const float* pnt2pixxyndcz,
+                                    const float* pnt2pixrad, const float* pnt2rgb,
+                                    float* pix2unitdepth, float* pix2rgb, uint32_t img_w,
+                                    uint32_t img_h, uint64_t* pix2key);
+
+/* Point painter on the GPU: the result of the reference CPU function
+ * render_pix_sort_depth (del-splat-cpu/src/pnt2pixxyndcz.rs:1-40; the GPU
+ * branch of point_splat_rasterize_pix is todo!() in the reference,
+ * del-splat-dlpack/src/splat_point.rs:40). Per pixel the point with the
+ * largest (ndc_z+1, index) wins, pixel = saturating cast of (x, y). Uncovered
+ * pixels are 0. pix2key u64 [H,W] is caller-owned scratch. */
+int dsplat_point_rasterize_pix(void* stream, uint32_t num_pnt, const float* pnt2pixxyndcz,
+                               const float* pnt2rgb, float* pix2rgb, uint32_t img_w,
+                               uint32_t img_h, uint64_t* pix2key);
+
+/* ======================================================================
+ * GPU primitives (replacing del-cudarc-sys)
+ * ====================================================================== */
+
+/* Workspace bytes of dsplat_exclusive_scan_u32 for n elements (host). */
+size_t dsplat_exclusive_scan_workspace_size(uint32_t n);
+
+/* out[i] = sum_{j<i} in[i] (u32, wrapping); in == out allowed.
+ * Replaces del_cudarc_sys::cumsum::exclusive_scan (splat_gauss.rs:137-146). */
+int dsplat_exclusive_scan_u32(void* stream, uint32_t n, const uint32_t* in, uint32_t* out,
+                              void* workspace, size_t workspace_bytes);
+
+/* Workspace bytes of dsplat_radix_sort_by_key_u64_u32 for n elements (host). */
+size_t dsplat_radix_sort_workspace_size(uint32_t n);
+
+/* Stable ascending sort of keys u64[n] carrying vals u32[n], in place. Only
+ * the low `key_bits` bits of the keys are compared (1..64; higher bits must be
+ * zero). Replaces del_cudarc_sys::sort_by_key_u64::radix_sort_by_key_u64_u32
+ * (splat_gauss.rs:184-191) and radix_sort_by_key_u64 (splat_circle.rs:86-90). */
+int dsplat_radix_sort_by_key_u64_u32(void* stream, uint32_t n, uint64_t* keys, uint32_t* vals,
+                                     uint32_t key_bits, void* workspace,
+                                     size_t workspace_bytes);
+
+/* buf[0..n) = value. Replaces del_cudarc_sys::array1d::fill_f32
+ * (del-splat-dlpack/src/splat_point.rs:86-87). */
+int dsplat_fill_f32(void* stream, uint32_t n, float* buf, float value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSPLAT_H_ */

@@ -1,0 +1,258 @@
+"""GPU parity of the Gaussian path against the oracle (same seeded inputs).
+
+Bars: projection bit-exact (both sides round every op in the same order;
+tolerance conic 1e-5 relative as the fallback bar in the assert message);
+binning / sort / tile ranges bit-exact, checked as the reference checks them
+(del-splat-cudarc/examples/03_splat_gauss.rs:124-153: the CPU binning is fed
+the GPU's projected boxes and depths); pixels within 1e-4 except at pixels
+where the two sides' 0.999 termination decision differs by rounding (counted,
+bounded, and each one explained); gradients within 2e-3 of the largest
+magnitude.
+"""
+import numpy as np
+import pytest
+import torch
+
+from _helpers import max_rel, small_scene, u32
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def _project(s, m, shape):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    return SG.pnt2splat3_to_pnt2splat2(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]),
+                                       _t(m), shape)
+
+
+@pytest.mark.parametrize("n,shape,seed", [(5000, (256, 192), 0), (20000, (333, 201), 1)])
+def test_projection_parity(oracle, n, shape, seed):
+    s, m = small_scene(n, seed, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    o_pxd, o_conic, o_aabb, nclamp = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m,
+                                                          shape)
+    g_pxd, g_conic, g_aabb = (t.cpu().numpy() for t in (pxd, conic, aabb))
+    assert nclamp == 0
+    assert max_rel(g_conic, o_conic) <= 1e-5
+    np.testing.assert_array_equal(g_pxd, o_pxd)
+    np.testing.assert_array_equal(g_conic, o_conic)
+    np.testing.assert_array_equal(g_aabb, o_aabb)
+
+
+@pytest.mark.parametrize("ts", [8, 16, 32])
+@pytest.mark.parametrize("n,shape,seed", [(3000, (256, 192), 2), (20000, (301, 177), 3),
+                                          (0, (64, 48), 4), (1, (64, 48), 5)])
+def test_binning_bit_exact(oracle, ts, n, shape, seed):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    s, m = small_scene(max(n, 1), seed, shape)
+    if n == 0:
+        s = {k: v[:0] for k, v in s.items()}
+    pxd, conic, aabb = _project(s, m, shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, ts)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+
+
+def test_binning_edge_boxes(oracle):
+    """Boxes off-screen, straddling every border, covering the whole image,
+    inverted (culled), NaN and exactly on tile boundaries."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape, ts = (100, 70), 16
+    boxes = np.array([
+        [-50, -50, -10, -10], [110, 10, 200, 20], [-5, -5, 5, 5], [95, 65, 130, 90],
+        [-1e9, -1e9, 1e9, 1e9], [np.inf, np.inf, -np.inf, -np.inf], [np.nan, 0, 10, 10],
+        [16, 16, 32, 32], [15.999, 0, 16.0, 0.0], [0, 0, 0, 0], [99.9, 69.9, 99.95, 69.95],
+    ], np.float32)
+    n = len(boxes)
+    pxd = np.zeros((n, 3), np.float32)
+    pxd[:, 2] = np.linspace(-1.5, 1.5, n).astype(np.float32)
+    pxd[3, 2] = pxd[4, 2]  # equal depths: ties by index
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, _t(pxd), _t(boxes))
+    o_t2i, o_i2p = oracle.tile2pnt(boxes, pxd, shape, ts)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+
+
+def test_binning_allocating_variant(oracle):
+    """dsplat_gauss_tile2idx_idx2pnt (library-owned idx2pnt, SYNC) == _ws."""
+    import ctypes
+    from del_splat_dlpack import _native
+    s, m = small_scene(4000, 6, (200, 150))
+    pxd, conic, aabb = _project(s, m, (200, 150))
+    tw, th = (200 + 15) // 16, (150 + 15) // 16
+    t2i = torch.empty(tw * th + 1, device=DEV, dtype=torch.int32)
+    out = ctypes.c_void_p()
+    k = ctypes.c_uint32()
+    stream = torch.cuda.current_stream().cuda_stream
+    _native.call("dsplat_gauss_tile2idx_idx2pnt", stream, 200, 150, 16, 4000, pxd.data_ptr(),
+                 aabb.data_ptr(), t2i.data_ptr(), ctypes.byref(out), ctypes.byref(k))
+    torch.cuda.synchronize()
+    buf = torch.empty(k.value, device=DEV, dtype=torch.int32)
+    _copy_device(buf.data_ptr(), out.value, 4 * k.value)
+    _native.call("dsplat_free", stream, out.value)
+    torch.cuda.synchronize()
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), (200, 150), 16)
+    assert k.value == len(o_i2p)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(buf), o_i2p)
+
+
+def _copy_device(dst, src, nbytes):
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+
+
+def test_capacity_overflow_reports_k(oracle):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    s, m = small_scene(2000, 7, (128, 128))
+    pxd, conic, aabb = _project(s, m, (128, 128))
+    kbuf = torch.zeros(4, device=DEV, dtype=torch.int32)
+    t2i, i2p = SG._bin_with_capacity((128, 128), 16, 2000, pxd, aabb, 10, kbuf)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), (128, 128), 16)
+    assert int(kbuf[0].item()) == len(o_i2p) > 10
+    assert int(t2i.abs().sum().item()) == 0  # nothing binned on overflow
+
+
+@pytest.mark.parametrize("n", [1, 1000, 2048, 100_003])
+@pytest.mark.parametrize("bits", [1, 13, 37, 64])
+def test_radix_sort_stable(n, bits):
+    from del_splat_dlpack import _native
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 2**63, n, dtype=np.uint64)
+    if bits == 64:
+        keys[: n // 2] |= np.uint64(1 << 63)
+    else:
+        keys &= np.uint64((1 << bits) - 1)
+    keys[rng.integers(0, n, n // 3 + 1)] = keys[0]  # many ties
+    vals = np.arange(n, dtype=np.uint32)
+    k = torch.from_numpy(keys.view(np.int64).copy()).to(DEV)
+    v = torch.from_numpy(vals.view(np.int32).copy()).to(DEV)
+    L = _native.lib()
+    wsb = L.dsplat_radix_sort_workspace_size(n)
+    ws = torch.empty(wsb, device=DEV, dtype=torch.uint8)
+    _native.call("dsplat_radix_sort_by_key_u64_u32", torch.cuda.current_stream().cuda_stream, n,
+                 k.data_ptr(), v.data_ptr(), bits, ws.data_ptr(), wsb)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k.cpu().numpy().view(np.uint64), keys[order])
+    np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), vals[order])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 1_000_001])
+def test_exclusive_scan(n):
+    from del_splat_dlpack import _native
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 100, n).astype(np.uint32)
+    t = torch.from_numpy(a.view(np.int32).copy()).to(DEV) if n else torch.zeros(1, device=DEV, dtype=torch.int32)
+    out = torch.empty_like(t)
+    L = _native.lib()
+    wsb = L.dsplat_exclusive_scan_workspace_size(n)
+    ws = torch.empty(wsb, device=DEV, dtype=torch.uint8)
+    _native.call("dsplat_exclusive_scan_u32", torch.cuda.current_stream().cuda_stream, n,
+                 t.data_ptr(), out.data_ptr(), ws.data_ptr(), wsb)
+    if n:
+        ref = np.concatenate([[0], np.cumsum(a, dtype=np.uint64)[:-1]]).astype(np.uint32)
+        np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref)
+
+
+def _check_pixels(g_img, g_alpha, g_nwalk, o_img, o_alpha, o_nwalk, tol=1e-4):
+    """Pixels agree within tol; where the two sides' termination differs the
+    oracle's alpha must sit within 1e-5 of the 0.999 threshold."""
+    diff = np.abs(g_img - o_img).max(axis=2)
+    flip = g_nwalk != o_nwalk
+    bad = (diff > tol) & ~flip
+    assert not bad.any(), f"{bad.sum()} pixels off by up to {diff[bad].max()}"
+    if flip.any():
+        # a flip can only happen when the running alpha crosses 0.999 within
+        # rounding distance on one side
+        near = np.abs(np.maximum(g_alpha, o_alpha) - 0.999) < 2e-3
+        assert near[flip].all()
+        assert flip.mean() < 1e-3, flip.mean()
+    assert np.abs(g_alpha - o_alpha)[~flip].max(initial=0) <= tol
+    return int(flip.sum())
+
+
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_raster_parity(oracle, ts):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (320, 240)
+    s, m = small_scene(30000, 10 + ts, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    op, rgb = _t(s["opacity"]), _t(s["rgb"])
+    img = SG.rasterize_pnt2splat2(shape, pxd, conic, aabb, op, rgb, ts, t2i, i2p).cpu().numpy()
+    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+        shape, pxd.cpu().numpy(), conic.cpu().numpy(), aabb.cpu().numpy(), s["opacity"],
+        s["rgb"], ts, u32(t2i), u32(i2p))
+    # packed-record path with the adjoint's state
+    _, _, _, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op, rgb,
+                                     _t(m), shape, log_scale=False)
+    for thr in ([0, 64, 128] if ts == 16 else [0, 512, 1024] if ts == 32 else [0]):
+        g_img, g_alpha, g_nwalk, g_tlast = (t.cpu().numpy() for t in SG.rasterize_fwd(
+            shape, ts, splat2, t2i, i2p, thr))
+        flips = _check_pixels(g_img, g_alpha, g_nwalk.view(np.uint32), o_img, o_alpha, o_nwalk)
+        print(f"ts={ts} threads={thr} threshold flips={flips}")
+        np.testing.assert_allclose(g_img, img, atol=1e-6)
+    assert (o_alpha > 0.999).mean() > 0.05  # the scene does saturate pixels
+
+
+def test_raster_matches_naive_renderer(oracle):
+    """Tiled GPU raster == the reference's naive non-tiled renderer on a tiny
+    scene whose depths stay inside (-1, 1) (pnt2pixxyndcz.rs:42-100)."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (64, 48)
+    s, m = small_scene(200, 11, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, pxd, aabb)
+    img = SG.rasterize_pnt2splat2(shape, pxd, conic, aabb, _t(s["opacity"]), _t(s["rgb"]), 16,
+                                  t2i, i2p).cpu().numpy()
+    naive = oracle.gauss_render_sort_depth(pxd.cpu().numpy(), conic.cpu().numpy(),
+                                           aabb.cpu().numpy(), s["opacity"], s["rgb"], shape)
+    assert np.abs(img - naive).max() <= 1e-4
+
+
+def _grad_case(oracle, n, shape, seed, ts, with_alpha):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    s, m = small_scene(n, seed, shape)
+    xyz, ls, q = _t(s["xyz"]), _t(s["log_scale"]), _t(s["quat"])
+    op, rgb = _t(s["opacity"]), _t(s["rgb"])
+    for t in (xyz, ls, q, op, rgb):
+        t.requires_grad_(True)
+    img, alpha = SG.render(xyz, ls, q, op, rgb, _t(m), shape, tile_size=ts)
+    rng = np.random.default_rng(seed + 100)
+    target = rng.random((shape[1], shape[0], 3)).astype(np.float32)
+    w_alpha = rng.random((shape[1], shape[0])).astype(np.float32) if with_alpha else None
+    loss = ((img - _t(target)) ** 2).sum()
+    if with_alpha:
+        loss = loss + (alpha * _t(w_alpha)).sum()
+    loss.backward()
+    # oracle chain on the same projection / lists
+    scale = np.exp(s["log_scale"]).astype(np.float32)
+    o_pxd, o_conic, o_aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], scale, m, shape)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(o_pxd, o_aabb, shape, ts)
+    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+        shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
+    d_img = 2.0 * (img.detach().cpu().numpy() - target)
+    dpix, dcon, dop, drgb = oracle.gauss_rasterize_bwd(
+        shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p, o_nwalk,
+        o_tlast, d_img, w_alpha)
+    dxyz, dls, dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], scale, m, shape, dpix, dcon,
+                                             log_scale=True)
+    return dict(xyz=(xyz.grad, dxyz), log_scale=(ls.grad, dls), quat=(q.grad, dq),
+                opacity=(op.grad, dop), rgb=(rgb.grad, drgb))
+
+
+@pytest.mark.parametrize("ts,with_alpha", [(16, False), (16, True), (8, False), (32, True)])
+def test_backward_matches_oracle_adjoint(oracle, ts, with_alpha):
+    res = _grad_case(oracle, 3000, (160, 120), 20 + ts, ts, with_alpha)
+    for name, (g, o) in res.items():
+        err = max_rel(g.cpu().numpy(), o)
+        print(f"{name}: max rel err {err:.2e}")
+        assert err <= 2e-3, (name, err)
