@@ -1,14 +1,21 @@
-"""Benchmark: differentiable 3D Gaussian splatting, forward + backward per frame.
+"""Benchmark: differentiable 3D Gaussian splatting on one MI355X.
 
-Workload (one step = one frame): 1M seeded Gaussians (means uniform in a
+Headline (one step = one frame): 1M seeded Gaussians (means uniform in a
 radius-2 ball, log-scales N(-4,0.5), random unit quaternions, opacity
 sigmoid(N(0,1)), colours U[0,1]), camera at distance 3, fov 60 deg, 1920x1080,
 16x16 tiles; loss = sum (rgb - target)^2 against a seeded U[0,1] image. A step
 runs projection, binning + stable sort, tiled raster, the loss gradient, the
-raster adjoint and the projection adjoint, all on one GPU, inputs resident in
-HBM. Prints one JSON line.
+raster adjoint and the projection adjoint, inputs resident in HBM.
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu-baseline]
+Batched section: the same 1M Gaussians seen by 16 cameras on a radius-3
+sphere with resolutions drawn from {640x480, 1280x720, 1920x1080, 800x800};
+forward + L2 adjoint of all 16 views as one graph-captured batched launch
+versus the per-view path (the single-view pipeline called once per view);
+and forward-only on 2M Gaussians in the unit ball. Tile size and waves per
+tile of the batched path come from the autotuner.
+
+Prints one JSON line.
+  python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu-baseline] [--no-batched]
 """
 import argparse
 import json
@@ -32,7 +39,10 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--views", type=int, default=16)
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     return ap.parse_args()
 
 
@@ -59,17 +69,7 @@ def cpu_baseline(s, m, shape, ts):
                       "fwd+bwd) of the same workload, oracle C++ -O3, 1 thread"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
-    import torch
-    import del_splat_dlpack.SplatGauss.torch as SG
-    from del_splat_dlpack import scene
-
+def headline(args, torch, np, SG, scene):
     dev = torch.device("cuda:0")
     shape = (args.width, args.height)
     ts = args.tile
@@ -79,15 +79,12 @@ def main():
     xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
     m16 = t(m)
     target = t(scene.target_image(shape, 102))
-
     stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
     ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for k in stages}
     acc = {k: 0.0 for k in stages}
-    kpairs = []
 
     def step(timed):
-        r = {}
         def mark(name, first):
             if timed:
                 ev[name][0 if first else 1].record()
@@ -108,26 +105,110 @@ def main():
                                                  d_img, None, args.threads)
         mark("raster_bwd", False)
         mark("project_bwd", True)
-        dxyz, dls, dq = SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
+        SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
         mark("project_bwd", False)
-        r["K"] = i2p.numel()
-        return r
+        return i2p.numel()
 
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    K = 0
     for _ in range(args.steps):
-        r = step(True)
-        kpairs.append(r["K"])
+        K = step(True)
         torch.cuda.synchronize()
         for k in stages:
             acc[k] += ev[k][0].elapsed_time(ev[k][1])
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = elapsed * 1000.0 / args.steps
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
     stage_ms = {k: acc[k] / args.steps for k in stages}
-    K = kpairs[-1]
+    return s, m, shape, ms, stage_ms, K
+
+
+def batched(args, torch, np, SG, scene):
+    """Per-view path vs one graph-captured batched launch over 16 uneven views."""
+    from del_splat_dlpack import autotune
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan, ensure_capacity
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    out = {}
+    reps = args.batch_reps
+    for mode, n, radius, seed in (("fwd_bwd", args.num_gauss, 2.0, 2),
+                                  ("forward", 2_000_000, 1.0, 3)):
+        s = scene.gaussians(n, seed, radius=radius)
+        views = scene.sphere_cameras(args.views, 3)
+        xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+        targets = [t(scene.target_image(shape, 200 + v)) for v, (_, shape) in enumerate(views)]
+        tflat = torch.cat([x.reshape(-1) for x in targets])
+        mats = [t(mm) for mm, _ in views]
+
+        def per_view():
+            for v, (_, shape) in enumerate(views):
+                pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, mats[v], shape)
+                t2i, i2p = SG.bin_gauss(shape, 16, pxd, aabb)
+                img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, 16, splat2, t2i, i2p)
+                if mode == "fwd_bwd":
+                    d_img = 2.0 * (img - targets[v])
+                    dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, 16, splat2, t2i, i2p, nwalk,
+                                                             tlast, d_img)
+                    SG.project_bwd(xyz, ls, q, mats[v], shape, dpix, dcon)
+
+        per_view()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            per_view()
+        torch.cuda.synchronize()
+        pv = args.views * reps / (time.perf_counter() - t0)
+
+        (ts, th), tune = autotune.tune_multiview(n, views, (xyz, ls, q, op, rgb), mode=mode,
+                                                 target=tflat, reps=2)
+
+        def run(p):
+            o, _ = p.forward(xyz, ls, q, op, rgb)
+            if mode == "fwd_bwd":
+                p.backward(xyz, ls, q, op, out_rgb=o, target=tflat)
+            return o
+
+        plan, _ = ensure_capacity(
+            lambda cap: MultiViewPlan(n, views, ts, th, pair_capacity=cap), run)
+        o_rgb = plan.new_image(3)
+        o_a = plan.new_image(1)
+        grads = torch.empty(14 * n, device=dev)
+        if mode == "fwd_bwd":
+            plan.capture_fwd_bwd(xyz, ls, q, op, rgb, o_rgb, tflat, grads)
+        else:
+            plan.capture_forward(xyz, ls, q, op, rgb, o_rgb, o_a)
+        plan.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.replay()
+        torch.cuda.synchronize()
+        bv = args.views * reps / (time.perf_counter() - t0)
+        k, _ = plan.counters()
+        plan.close()
+        torch.cuda.empty_cache()
+        out[mode] = {"num_gauss": n, "views": args.views,
+                     "resolutions": [f"{w}x{h}" for _, (w, h) in views],
+                     "per_view_views_per_s": round(pv, 2), "batched_views_per_s": round(bv, 2),
+                     "speedup": round(bv / pv, 3), "tile": ts, "threads": th, "pairs_K": k,
+                     "autotune_ms": {kk: round(vv, 3) for kk, vv in tune.items()}}
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import scene
+
+    s, m, shape, ms, stage_ms, K = headline(args, torch, np, SG, scene)
     N = args.num_gauss
     P = shape[0] * shape[1]
     # algorithmic bytes per launch (implementation independent, reference layouts)
@@ -159,16 +240,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Gaussians and target image)",
-        "config": {"workload": f"3dgs fwd+bwd, {N} Gaussians, {shape[0]}x{shape[1]}, tile {ts}",
-                   "num_gauss": N, "width": shape[0], "height": shape[1], "tile": ts,
-                   "pairs_K": K, "parallelism": "single GPU"},
+        "config": {"workload": f"3dgs fwd+bwd, {N} Gaussians, {shape[0]}x{shape[1]}, tile "
+                               f"{args.tile}", "num_gauss": N, "width": shape[0],
+                   "height": shape[1], "tile": args.tile, "pairs_K": K,
+                   "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    if not args.no_batched:
+        out["batched"] = batched(args, torch, np, SG, scene)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s, m, shape, ts)
+        out["cpu_baseline"] = cpu_baseline(s, m, shape, args.tile)
     print(json.dumps(out), flush=True)
 
 

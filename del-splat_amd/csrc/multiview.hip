@@ -1,0 +1,733 @@
+// Batched multi-view rendering and differentiation over one shared set of
+// Gaussians (no counterpart in the reference, which renders one view per
+// call: del-splat-cudarc/src/splat_gauss.rs).
+//
+// A plan owns every device buffer for N Gaussians and up to V views, so a
+// frame is a fixed sequence of kernels with no allocation and no host
+// synchronisation; it can be captured into a hipGraph and replayed.
+//
+// Items: (view v, Gaussian i) -> item v*N + i. Per item the projection kernel
+// writes the packed raster record and the binning counters in one pass. The
+// binning is the single-view algorithm applied to all items at once: one
+// stable sort of the V*N depth keys, pair generation with *global* tile ids
+// (view tile offset + local tile), one stable sort by global tile id. Within
+// a (view, tile) the order is (depth bits, Gaussian index), the reference's
+// per-view order (tile_acceleration.h:15-52 + stable u64 sort).
+//
+// Raster and adjoint run as one persistent launch each: workgroups pull
+// global tiles from an atomic counter until all tiles of all views are done,
+// so views of uneven resolution share one launch without tail gaps between
+// per-view grids.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "primitives.hpp"
+#include "project_math.hpp"
+#include "raster_core.hpp"
+
+namespace dsplat {
+
+struct ViewDev {
+  float m[16];
+  uint32_t w, h, tw, th;
+  uint32_t tile_off, pad0;
+  uint64_t pix_off;
+};
+
+constexpr uint32_t kMaxViews = 64;
+
+// Finds the view of a global tile (views are few; the scan is uniform across
+// the workgroup).
+__device__ inline uint32_t view_of_tile(const ViewDev* __restrict__ views, uint32_t nview,
+                                        uint32_t g) {
+  uint32_t v = 0;
+  while (v + 1 < nview && views[v + 1].tile_off <= g) ++v;
+  return v;
+}
+
+__device__ inline float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
+
+// ---------------------------------------------------------- projection ---
+template <bool kLogScale, bool kLogitOp>
+__global__ void __launch_bounds__(256)
+mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
+                  const float* __restrict__ xyz, const float* __restrict__ scale,
+                  const float4* __restrict__ quat, const float* __restrict__ opacity,
+                  const float* __restrict__ rgb, float4* __restrict__ rec,
+                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ dkey,
+                  uint32_t* __restrict__ order) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = blockIdx.y;
+  const ViewDev& vd = views[v];
+  const float4 q = quat[i];
+  float s0 = scale[3 * i], s1 = scale[3 * i + 1], s2 = scale[3 * i + 2];
+  if (kLogScale) {
+    s0 = expf(s0);
+    s1 = expf(s1);
+    s2 = expf(s2);
+  }
+  ProjState p;
+  project_state(vd.m, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q.x, q.y, q.z, q.w, s0, s1,
+                s2, vd.w, vd.h, p);
+  float op = opacity[i];
+  if (kLogitOp) op = sigmoidf_(op);
+  const size_t item = size_t(v) * n + i;
+  float4* r = rec + 4 * item;
+  r[0] = make_float4(p.pix[0], p.pix[1], p.conic[0], p.conic[1]);
+  r[1] = make_float4(p.conic[2], op, rgb[3 * i], rgb[3 * i + 1]);
+  r[2] = make_float4(rgb[3 * i + 2], p.aabb[0], p.aabb[1], p.aabb[2]);
+  r[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
+  const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
+  cnt[item] = uint32_t(tr.count());
+  dkey[item] = depth_key(p.ndcz);
+  order[item] = uint32_t(item);
+}
+
+// ------------------------------------------------------------- binning ---
+__global__ void __launch_bounds__(256)
+mv_gather_kernel(uint32_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ src,
+                 uint32_t* __restrict__ dst) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) dst[j] = src[order[j]];
+}
+
+__global__ void mv_clamp_kernel(uint32_t* __restrict__ misc, uint32_t capacity) {
+  const uint32_t k = misc[0];
+  misc[1] = k <= capacity ? k : 0u;
+  misc[2] = 0u;  // forward work counter
+  misc[3] = 0u;  // adjoint work counter
+  misc[4] = k > capacity ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
+               const uint32_t* __restrict__ order, const uint32_t* __restrict__ off,
+               const uint32_t* __restrict__ misc, const float4* __restrict__ rec,
+               uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ item_of_pair) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nitems) return;
+  if (misc[1] == 0) return;
+  const uint32_t item = order[j];
+  const uint32_t v = item / n;
+  const ViewDev& vd = views[v];
+  const float4 q2 = rec[4 * size_t(item) + 2];
+  const float4 q3 = rec[4 * size_t(item) + 3];
+  const TileRect r = tile_rect(q2.y, q2.z, q2.w, q3.x, ts, vd.tw, vd.th);
+  uint32_t o = off[j];
+  for (int ix = r.x0; ix < r.x1; ++ix)
+    for (int iy = r.y0; iy < r.y1; ++iy) {
+      tile_of_pair[o] = vd.tile_off + uint32_t(iy) * vd.tw + uint32_t(ix);
+      item_of_pair[o] = item;
+      ++o;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+mv_ranges_kernel(const uint32_t* __restrict__ misc, uint32_t capacity, uint32_t num_tile,
+                 const uint32_t* __restrict__ tile_sorted, uint32_t* __restrict__ tile2idx) {
+  const uint32_t k = misc[1];
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx > k || idx > capacity) return;
+  const uint32_t lo = idx == 0 ? 0u : tile_sorted[idx - 1] + 1u;
+  const uint32_t hi = idx == k ? num_tile : tile_sorted[idx];
+  for (uint32_t t = lo; t <= hi; ++t) tile2idx[t] = idx;
+}
+
+__global__ void __launch_bounds__(256)
+mv_copy_kernel(const uint32_t* __restrict__ misc, uint32_t cap, const uint32_t* __restrict__ src,
+               uint32_t* __restrict__ dst) {
+  const uint32_t n = min(misc[1], cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256)
+zero_f32_kernel(size_t n, float* __restrict__ p) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = 0.f;
+}
+
+// -------------------------------------------------------------- raster ---
+template <int TS, int NT>
+__global__ void __launch_bounds__(NT)
+mv_raster_fwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t total_tiles,
+                     const float4* __restrict__ rec, const uint32_t* __restrict__ tile2idx,
+                     const uint32_t* __restrict__ idx2pnt, float* __restrict__ rgb,
+                     float* __restrict__ alpha, uint32_t* __restrict__ nwalk,
+                     float* __restrict__ tlast, uint32_t* __restrict__ counter) {
+  __shared__ uint32_t s_item;
+  const PackedLoader ld{rec};
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t g = s_item;
+    __syncthreads();
+    if (g >= total_tiles) break;
+    const uint32_t v = view_of_tile(views, nview, g);
+    const ViewDev& vd = views[v];
+    const uint32_t local = g - vd.tile_off;
+    const size_t po = vd.pix_off;
+    raster_tile_fwd<TS, NT, PackedLoader>(local % vd.tw, local / vd.tw, vd.w, vd.h, tile2idx[g],
+                                          tile2idx[g + 1], ld, idx2pnt, rgb + 3 * po, alpha + po,
+                                          nwalk + po, tlast + po);
+  }
+}
+
+template <int TS, int NT>
+__global__ void __launch_bounds__(NT)
+mv_raster_bwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t n,
+                     uint32_t total_tiles, const float4* __restrict__ rec,
+                     const uint32_t* __restrict__ tile2idx, const uint32_t* __restrict__ idx2pnt,
+                     const uint32_t* __restrict__ nwalk, const float* __restrict__ tlast,
+                     const float* __restrict__ d_img, const float* __restrict__ rgb,
+                     const float* __restrict__ target, const float* __restrict__ d_alpha,
+                     GradSink out, uint32_t* __restrict__ counter) {
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t g = s_item;
+    __syncthreads();
+    if (g >= total_tiles) break;
+    const uint32_t v = view_of_tile(views, nview, g);
+    const ViewDev& vd = views[v];
+    const uint32_t local = g - vd.tile_off;
+    const size_t po = vd.pix_off;
+    raster_tile_bwd<TS, NT>(local % vd.tw, local / vd.tw, vd.w, vd.h, tile2idx[g + 1], rec,
+                            idx2pnt, nwalk + po, tlast + po, d_img ? d_img + 3 * po : nullptr,
+                            rgb + 3 * po, target ? target + 3 * po : nullptr,
+                            d_alpha ? d_alpha + po : nullptr, v * n, out);
+  }
+}
+
+// Projection adjoint summed over views, one thread per Gaussian. Overwrites
+// d_xyz / d_scale / d_quat; with kLogitOp also turns dL/dopacity into
+// dL/dlogit in place.
+template <bool kLogScale, bool kLogitOp>
+__global__ void __launch_bounds__(256)
+mv_project_bwd_kernel(uint32_t n, uint32_t nview, const ViewDev* __restrict__ views,
+                      const float* __restrict__ xyz, const float* __restrict__ scale,
+                      const float4* __restrict__ quat, const float* __restrict__ opacity,
+                      const float2* __restrict__ d_pixxy, const float* __restrict__ d_conic,
+                      float* __restrict__ d_xyz, float* __restrict__ d_scale,
+                      float4* __restrict__ d_quat, float* __restrict__ d_op) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float sg[3] = {scale[3 * i], scale[3 * i + 1], scale[3 * i + 2]};
+  if (kLogScale) {
+    sg[0] = expf(sg[0]);
+    sg[1] = expf(sg[1]);
+    sg[2] = expf(sg[2]);
+  }
+  const float x3[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  const float4 qt = quat[i];
+  float ax[3] = {0.f, 0.f, 0.f}, as[3] = {0.f, 0.f, 0.f}, aq[4] = {0.f, 0.f, 0.f, 0.f};
+  for (uint32_t v = 0; v < nview; ++v) {
+    const size_t item = size_t(v) * n + i;
+    const float2 gp = d_pixxy[item];
+    const float gcon[3] = {d_conic[3 * item], d_conic[3 * item + 1], d_conic[3 * item + 2]};
+    if (gp.x == 0.f && gp.y == 0.f && gcon[0] == 0.f && gcon[1] == 0.f && gcon[2] == 0.f)
+      continue;
+    float dx[3], dsc[3], dq[4];
+    if (!project_bwd_one(views[v].m, views[v].w, views[v].h, x3, qt, sg, kLogScale, gp, gcon, dx,
+                         dsc, dq))
+      continue;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ax[k] += dx[k];
+      as[k] += dsc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) aq[k] += dq[k];
+  }
+  d_xyz[3 * i + 0] = ax[0];
+  d_xyz[3 * i + 1] = ax[1];
+  d_xyz[3 * i + 2] = ax[2];
+  d_scale[3 * i + 0] = as[0];
+  d_scale[3 * i + 1] = as[1];
+  d_scale[3 * i + 2] = as[2];
+  d_quat[i] = make_float4(aq[0], aq[1], aq[2], aq[3]);
+  if (kLogitOp) {
+    const float op = sigmoidf_(opacity[i]);
+    d_op[i] *= op * (1.f - op);
+  }
+}
+
+// ------------------------------------------------------------------ Adam ---
+// Kingma & Ba (2015), Algorithm 1, elementwise; the step counter lives in
+// device memory (incremented by adam_tick_kernel) so a captured graph can be
+// replayed step after step.
+__global__ void __launch_bounds__(256)
+adam_kernel(size_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+            float* __restrict__ m2, float lr, float b1, float b2, float eps,
+            const uint32_t* __restrict__ d_step) {
+  const float t = float(*d_step);
+  const float bc1 = 1.f - powf(b1, t);
+  const float bc2 = 1.f - powf(b2, t);
+  const float step_size = lr / bc1;
+  const float inv_sqrt_bc2 = rsqrtf(bc2);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    p[i] -= step_size * a / (sqrtf(b) * inv_sqrt_bc2 + eps);
+  }
+}
+
+__global__ void adam_tick_kernel(uint32_t* d_step) { *d_step += 1u; }
+
+__global__ void set_word_kernel(uint32_t* p, uint32_t v) { *p = v; }
+
+inline uint32_t grid_cap(size_t n) {
+  const size_t b = (n + 255) / 256;
+  return uint32_t(b > 8192 ? 8192 : (b ? b : 1));
+}
+
+}  // namespace dsplat
+
+using namespace dsplat;
+
+// ----------------------------------------------------------------- plan ---
+struct dsplat_mv_plan {
+  uint32_t n = 0, nview = 0, max_view = 0, ts = 16, threads = 0, capacity = 0;
+  uint32_t max_tiles = 0, total_tiles = 0;
+  uint64_t max_pix = 0, total_pix = 0;
+  uint32_t persist_fwd = 0, persist_bwd = 0;
+  std::vector<ViewDev> host_views;
+  char* mem = nullptr;
+  size_t mem_bytes = 0;
+  ViewDev* d_views = nullptr;
+  float4* rec = nullptr;
+  uint32_t *cnt = nullptr, *dkey = nullptr, *order = nullptr;
+  void* dsort_ws = nullptr;
+  size_t dsort_bytes = 0;
+  uint32_t *off = nullptr, *scan_ws = nullptr;
+  uint32_t *tilekey = nullptr, *idx2pnt = nullptr;
+  void* tsort_ws = nullptr;
+  size_t tsort_bytes = 0;
+  uint32_t *tile2idx = nullptr, *misc = nullptr;
+  float *rgb = nullptr, *alpha = nullptr, *tlast = nullptr;
+  uint32_t* nwalk = nullptr;
+  float *d_pixxy = nullptr, *d_conic = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap_stream = nullptr;
+};
+
+static size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
+
+static int fill_views(dsplat_mv_plan* P, uint32_t nview, const dsplat_view_desc* views) {
+  if (nview == 0 || nview > P->max_view)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: view count out of range");
+  std::vector<ViewDev> vd(nview);
+  uint32_t toff = 0;
+  uint64_t poff = 0;
+  for (uint32_t v = 0; v < nview; ++v) {
+    memcpy(vd[v].m, views[v].world2ndc, sizeof(vd[v].m));
+    vd[v].w = views[v].img_w;
+    vd[v].h = views[v].img_h;
+    vd[v].tw = (views[v].img_w + P->ts - 1) / P->ts;
+    vd[v].th = (views[v].img_h + P->ts - 1) / P->ts;
+    vd[v].tile_off = toff;
+    vd[v].pad0 = 0;
+    vd[v].pix_off = poff;
+    toff += vd[v].tw * vd[v].th;
+    poff += uint64_t(views[v].img_w) * views[v].img_h;
+  }
+  if (toff > P->max_tiles || poff > P->max_pix)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "multiview: views exceed the plan's pixels/tiles");
+  P->host_views = vd;
+  P->nview = nview;
+  P->total_tiles = toff;
+  P->total_pix = poff;
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64_t max_pixels,
+                                     uint32_t tile_size, uint32_t threads, uint32_t pair_capacity,
+                                     dsplat_mv_plan** out) {
+  if (!out) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan pointer");
+  *out = nullptr;
+  if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "tile_size must be 8, 16 or 32");
+  if (max_view == 0 || max_view > kMaxViews || num_pnt == 0 || max_pixels == 0)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: empty or too many views / points");
+  if (uint64_t(num_pnt) * max_view >= (1ull << 32))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: num_pnt * views must fit in 32 bits");
+  dsplat_mv_plan* P = new dsplat_mv_plan();
+  P->n = num_pnt;
+  P->max_view = max_view;
+  P->ts = tile_size;
+  P->threads = threads;
+  P->capacity = pair_capacity;
+  P->max_pix = max_pixels;
+  // each view adds at most one partial tile row and column
+  const uint64_t tiles = max_pixels / (uint64_t(tile_size) * tile_size) +
+                         uint64_t(max_view) * (2 * 4096 / tile_size + 1);
+  P->max_tiles = uint32_t(tiles > 0xFFFFFFF0ull ? 0xFFFFFFF0ull : tiles);
+  const size_t items = size_t(num_pnt) * max_view;
+  size_t o = 0;
+  auto take = [&](size_t b) { size_t r = o; o += a256(b); return r; };
+  const size_t o_views = take(sizeof(ViewDev) * max_view);
+  const size_t o_rec = take(items * 64);
+  const size_t o_cnt = take(items * 4), o_dkey = take(items * 4), o_order = take(items * 4);
+  P->dsort_bytes = sort_workspace_bytes<uint32_t>(uint32_t(items));
+  const size_t o_dsort = take(P->dsort_bytes);
+  const size_t o_off = take((items + 1) * 4);
+  const size_t o_scan = take(scan_workspace_bytes(uint32_t(items)));
+  const size_t o_tkey = take(size_t(pair_capacity) * 4 + 4);
+  const size_t o_i2p = take(size_t(pair_capacity) * 4 + 4);
+  P->tsort_bytes = sort_workspace_bytes<uint32_t>(pair_capacity);
+  const size_t o_tsort = take(P->tsort_bytes);
+  const size_t o_t2i = take((size_t(P->max_tiles) + 1) * 4);
+  const size_t o_misc = take(64);
+  const size_t o_rgb = take(max_pixels * 12), o_alpha = take(max_pixels * 4);
+  const size_t o_nwalk = take(max_pixels * 4), o_tlast = take(max_pixels * 4);
+  const size_t o_dpix = take(items * 8), o_dcon = take(items * 12);
+  P->mem_bytes = o;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&P->mem), P->mem_bytes);
+  if (e != hipSuccess) {
+    delete P;
+    return dsplat_set_hip_error(e, "multiview: hipMalloc plan memory");
+  }
+  char* b = P->mem;
+  P->d_views = reinterpret_cast<ViewDev*>(b + o_views);
+  P->rec = reinterpret_cast<float4*>(b + o_rec);
+  P->cnt = reinterpret_cast<uint32_t*>(b + o_cnt);
+  P->dkey = reinterpret_cast<uint32_t*>(b + o_dkey);
+  P->order = reinterpret_cast<uint32_t*>(b + o_order);
+  P->dsort_ws = b + o_dsort;
+  P->off = reinterpret_cast<uint32_t*>(b + o_off);
+  P->scan_ws = reinterpret_cast<uint32_t*>(b + o_scan);
+  P->tilekey = reinterpret_cast<uint32_t*>(b + o_tkey);
+  P->idx2pnt = reinterpret_cast<uint32_t*>(b + o_i2p);
+  P->tsort_ws = b + o_tsort;
+  P->tile2idx = reinterpret_cast<uint32_t*>(b + o_t2i);
+  P->misc = reinterpret_cast<uint32_t*>(b + o_misc);
+  P->rgb = reinterpret_cast<float*>(b + o_rgb);
+  P->alpha = reinterpret_cast<float*>(b + o_alpha);
+  P->nwalk = reinterpret_cast<uint32_t*>(b + o_nwalk);
+  P->tlast = reinterpret_cast<float*>(b + o_tlast);
+  P->d_pixxy = reinterpret_cast<float*>(b + o_dpix);
+  P->d_conic = reinterpret_cast<float*>(b + o_dcon);
+  // persistent grids: resident workgroups per CU times CUs
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 256;
+  P->persist_fwd = uint32_t(cus > 0 ? cus : 256) * 8;
+  P->persist_bwd = uint32_t(cus > 0 ? cus : 256) * 4;
+  *out = P;
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_plan_destroy(dsplat_mv_plan* P) {
+  if (!P) return DSPLAT_OK;
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
+  if (P->mem) (void)hipFree(P->mem);
+  delete P;
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num_view,
+                                   const dsplat_view_desc* views) {
+  if (!P || !views) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan/views");
+  if (int rc = fill_views(P, num_view, views)) return rc;
+  DSPLAT_HIP(hipMemcpyAsync(P->d_views, P->host_views.data(), sizeof(ViewDev) * num_view,
+                            hipMemcpyHostToDevice, as_stream(stream)));
+  DSPLAT_HIP(hipStreamSynchronize(as_stream(stream)));
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_plan_info(const dsplat_mv_plan* P, uint32_t* num_view,
+                                   uint64_t* total_pix, uint32_t* total_tiles,
+                                   uint64_t* pix_offsets, uint32_t* tile_offsets) {
+  if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
+  if (num_view) *num_view = P->nview;
+  if (total_pix) *total_pix = P->total_pix;
+  if (total_tiles) *total_tiles = P->total_tiles;
+  for (uint32_t v = 0; v < P->nview; ++v) {
+    if (pix_offsets) pix_offsets[v] = P->host_views[v].pix_off;
+    if (tile_offsets) tile_offsets[v] = P->host_views[v].tile_off;
+  }
+  if (pix_offsets) pix_offsets[P->nview] = P->total_pix;
+  if (tile_offsets) tile_offsets[P->nview] = P->total_tiles;
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_outputs(const dsplat_mv_plan* P, float** rgb, float** alpha,
+                                 uint32_t** nwalk, float** tlast, uint32_t** tile2idx,
+                                 uint32_t** idx2pnt, uint32_t** counters) {
+  if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
+  if (rgb) *rgb = P->rgb;
+  if (alpha) *alpha = P->alpha;
+  if (nwalk) *nwalk = P->nwalk;
+  if (tlast) *tlast = P->tlast;
+  if (tile2idx) *tile2idx = P->tile2idx;
+  if (idx2pnt) *idx2pnt = P->idx2pnt;
+  if (counters) *counters = P->misc;
+  return DSPLAT_OK;
+}
+
+static uint32_t bitw(uint32_t v) {
+  uint32_t b = 0;
+  while (v) { ++b; v >>= 1; }
+  return b;
+}
+
+template <int TS, int NT>
+static void launch_mv_fwd(dsplat_mv_plan* P, hipStream_t s, float* out_rgb, float* out_alpha) {
+  mv_raster_fwd_kernel<TS, NT><<<P->persist_fwd, NT, 0, s>>>(
+      P->d_views, P->nview, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, out_rgb, out_alpha,
+      P->nwalk, P->tlast, P->misc + 2);
+}
+
+template <int TS, int NT>
+static void launch_mv_bwd(dsplat_mv_plan* P, hipStream_t s, const float* out_rgb,
+                          const float* d_img, const float* target, const float* d_alpha,
+                          GradSink out) {
+  mv_raster_bwd_kernel<TS, NT><<<P->persist_bwd, NT, 0, s>>>(
+      P->d_views, P->nview, P->n, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, P->nwalk,
+      P->tlast, d_img, out_rgb, target, d_alpha, out, P->misc + 3);
+}
+
+static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
+                      const float* quat, const float* opacity, const float* rgb,
+                      uint32_t flags, float* out_rgb, float* out_alpha) {
+  if (!out_alpha) out_alpha = P->alpha;
+  const uint32_t n = P->n, nv = P->nview;
+  const uint32_t items = n * nv;
+  const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, logit = flags & DSPLAT_OPACITY_IS_LOGIT;
+  const dim3 pg(div_up(n, 256), nv);
+  const float4* q4 = reinterpret_cast<const float4*>(quat);
+#define DSPLAT_MVP(A, B)                                                                    \
+  mv_project_kernel<A, B><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
+                                             rgb, P->rec, P->cnt, P->dkey, P->order)
+  if (log_scale && logit) DSPLAT_MVP(true, true);
+  else if (log_scale) DSPLAT_MVP(true, false);
+  else if (logit) DSPLAT_MVP(false, true);
+  else DSPLAT_MVP(false, false);
+#undef DSPLAT_MVP
+  bool in_alt = false;
+  uint32_t *kalt = nullptr, *valt = nullptr;
+  int rc = radix_sort_pairs<uint32_t>(s, items, nullptr, P->dkey, P->order, 31, P->dsort_ws,
+                                      P->dsort_bytes, &in_alt, &kalt, &valt);
+  if (rc) return rc;
+  const uint32_t* order = in_alt ? valt : P->order;
+  mv_gather_kernel<<<div_up(items, 256), 256, 0, s>>>(items, order, P->cnt, P->off);
+  rc = exclusive_scan(s, items, nullptr, P->off, P->off, P->scan_ws, P->misc);
+  if (rc) return rc;
+  mv_clamp_kernel<<<1, 1, 0, s>>>(P->misc, P->capacity);
+  mv_fill_kernel<<<div_up(items, 256), 256, 0, s>>>(items, n, P->d_views, P->ts, order, P->off,
+                                                     P->misc, P->rec, P->tilekey, P->idx2pnt);
+  const uint32_t* tsorted = P->tilekey;
+  if (P->capacity > 0 && P->total_tiles > 1) {
+    rc = radix_sort_pairs<uint32_t>(s, P->capacity, P->misc + 1, P->tilekey, P->idx2pnt,
+                                    bitw(P->total_tiles - 1), P->tsort_ws, P->tsort_bytes,
+                                    &in_alt, &kalt, &valt);
+    if (rc) return rc;
+    if (in_alt) {
+      mv_copy_kernel<<<1024, 256, 0, s>>>(P->misc, P->capacity, valt, P->idx2pnt);
+      tsorted = kalt;
+    }
+  }
+  mv_ranges_kernel<<<div_up(uint64_t(P->capacity) + 1, 256), 256, 0, s>>>(
+      P->misc, P->capacity, P->total_tiles, tsorted, P->tile2idx);
+  const uint32_t ts = P->ts, th = P->threads;
+  if (ts == 8) launch_mv_fwd<8, 64>(P, s, out_rgb, out_alpha);
+  else if (ts == 16 && th == 64) launch_mv_fwd<16, 64>(P, s, out_rgb, out_alpha);
+  else if (ts == 16 && th == 128) launch_mv_fwd<16, 128>(P, s, out_rgb, out_alpha);
+  else if (ts == 16) launch_mv_fwd<16, 256>(P, s, out_rgb, out_alpha);
+  else if (th == 1024) launch_mv_fwd<32, 1024>(P, s, out_rgb, out_alpha);
+  else if (th == 512) launch_mv_fwd<32, 512>(P, s, out_rgb, out_alpha);
+  else launch_mv_fwd<32, 256>(P, s, out_rgb, out_alpha);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
+                       const float* quat, const float* opacity, const float* out_rgb,
+                       const float* d_img, const float* target, const float* d_alpha,
+                       float* d_xyz, float* d_scale, float* d_quat, float* d_op, float* d_rgb,
+                       uint32_t flags) {
+  const uint32_t n = P->n, nv = P->nview;
+  const size_t items = size_t(n) * nv;
+  set_word_kernel<<<1, 1, 0, s>>>(P->misc + 3, 0u);
+  zero_f32_kernel<<<grid_cap(items * 2), 256, 0, s>>>(items * 2, P->d_pixxy);
+  zero_f32_kernel<<<grid_cap(items * 3), 256, 0, s>>>(items * 3, P->d_conic);
+  zero_f32_kernel<<<grid_cap(n), 256, 0, s>>>(n, d_op);
+  zero_f32_kernel<<<grid_cap(size_t(n) * 3), 256, 0, s>>>(size_t(n) * 3, d_rgb);
+  const GradSink out{P->d_pixxy, P->d_conic, d_op, d_rgb};
+  const uint32_t ts = P->ts, th = P->threads;
+  if (ts == 8) launch_mv_bwd<8, 64>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else if (ts == 16 && th == 64) launch_mv_bwd<16, 64>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else if (ts == 16 && th == 128) launch_mv_bwd<16, 128>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else if (ts == 16) launch_mv_bwd<16, 256>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else launch_mv_bwd<32, 256>(P, s, out_rgb, d_img, target, d_alpha, out);
+  const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, logit = flags & DSPLAT_OPACITY_IS_LOGIT;
+  const float4* q4 = reinterpret_cast<const float4*>(quat);
+  const float2* gp = reinterpret_cast<const float2*>(P->d_pixxy);
+  float4* dq = reinterpret_cast<float4*>(d_quat);
+#define DSPLAT_MVB(A, B)                                                                   \
+  mv_project_bwd_kernel<A, B><<<div_up(n, 256), 256, 0, s>>>(n, nv, P->d_views, xyz, scale, \
+                                                             q4, opacity, gp, P->d_conic,  \
+                                                             d_xyz, d_scale, dq, d_op)
+  if (log_scale && logit) DSPLAT_MVB(true, true);
+  else if (log_scale) DSPLAT_MVB(true, false);
+  else if (logit) DSPLAT_MVB(false, true);
+  else DSPLAT_MVB(false, false);
+#undef DSPLAT_MVB
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+extern "C" int dsplat_mv_forward(dsplat_mv_plan* P, void* stream, const float* xyz,
+                                 const float* scale, const float* quat, const float* opacity,
+                                 const float* rgb, uint32_t flags, float* out_rgb,
+                                 float* out_alpha) {
+  if (!P || !xyz || !scale || !quat || !opacity || !rgb || !out_rgb)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_forward: null argument");
+  if (P->nview == 0) return dsplat_set_error(DSPLAT_ERR_ARG, "mv_forward: no views set");
+  if (!al16(quat)) return dsplat_set_error(DSPLAT_ERR_ARG, "mv_forward: quat not 16-byte aligned");
+  return mv_forward(P, as_stream(stream), xyz, scale, quat, opacity, rgb, flags, out_rgb,
+                    out_alpha);
+}
+
+extern "C" int dsplat_mv_backward(dsplat_mv_plan* P, void* stream, const float* xyz,
+                                  const float* scale, const float* quat, const float* opacity,
+                                  const float* out_rgb, const float* d_img, const float* target,
+                                  const float* d_alpha, float* d_xyz, float* d_scale,
+                                  float* d_quat, float* d_opacity, float* d_rgb, uint32_t flags) {
+  if (!P || !xyz || !scale || !quat || !opacity || !d_xyz || !d_scale || !d_quat ||
+      !d_opacity || !d_rgb || (!d_img && (!target || !out_rgb)))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_backward: null argument");
+  if (!al16(quat) || !al16(d_quat))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_backward: quat buffers not 16-byte aligned");
+  return mv_backward(P, as_stream(stream), xyz, scale, quat, opacity, out_rgb, d_img, target,
+                     d_alpha, d_xyz, d_scale, d_quat, d_opacity, d_rgb, flags);
+}
+
+extern "C" int dsplat_memcpy_d2d(void* stream, void* dst, const void* src, uint64_t bytes) {
+  if (bytes == 0) return DSPLAT_OK;
+  if (!dst || !src) return dsplat_set_error(DSPLAT_ERR_ARG, "memcpy: null pointer");
+  DSPLAT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_adam_step(void* stream, uint64_t n, float* param, const float* grad,
+                                float* m1, float* m2, float lr, float beta1, float beta2,
+                                float eps, uint32_t* d_step) {
+  if (n == 0) return DSPLAT_OK;
+  if (!param || !grad || !m1 || !m2 || !d_step)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "adam: null argument");
+  hipStream_t s = as_stream(stream);
+  adam_tick_kernel<<<1, 1, 0, s>>>(d_step);
+  adam_kernel<<<grid_cap(n), 256, 0, s>>>(n, param, grad, m1, m2, lr, beta1, beta2, eps, d_step);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+// One training step over the plan's current views on a flat parameter block
+// [xyz 3N | log-scale 3N | quat 4N | opacity logit N | rgb 3N]: forward, the
+// adjoint of sum (rgb - target)^2 over all views, and Adam on all 14N values.
+static int mv_train_step(dsplat_mv_plan* P, hipStream_t s, float* params, float* grads,
+                         float* m1, float* m2, uint32_t* d_step, const float* targets,
+                         float* out_rgb, float lr, float b1, float b2, float eps) {
+  const size_t n = P->n;
+  float* xyz = params;
+  float* ls = params + 3 * n;
+  float* q = params + 6 * n;
+  float* op = params + 10 * n;
+  float* rgb = params + 11 * n;
+  const uint32_t flags = DSPLAT_SCALE_IS_LOG | DSPLAT_OPACITY_IS_LOGIT;
+  int rc = mv_forward(P, s, xyz, ls, q, op, rgb, flags, out_rgb, nullptr);
+  if (rc) return rc;
+  rc = mv_backward(P, s, xyz, ls, q, op, out_rgb, nullptr, targets, nullptr, grads,
+                   grads + 3 * n, grads + 6 * n, grads + 10 * n, grads + 11 * n, flags);
+  if (rc) return rc;
+  adam_tick_kernel<<<1, 1, 0, s>>>(d_step);
+  adam_kernel<<<grid_cap(14 * n), 256, 0, s>>>(14 * n, params, grads, m1, m2, lr, b1, b2, eps,
+                                               d_step);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_train_step(dsplat_mv_plan* P, void* stream, float* params, float* grads,
+                                    float* m1, float* m2, uint32_t* d_step, const float* targets,
+                                    float* out_rgb, float lr, float beta1, float beta2,
+                                    float eps) {
+  if (!P || !params || !grads || !m1 || !m2 || !d_step || !targets || !out_rgb)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: null argument");
+  if (P->nview == 0) return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: no views set");
+  if (!al16(params + 6 * size_t(P->n)) || !al16(grads + 6 * size_t(P->n)))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: quat block not 16-byte aligned");
+  return mv_train_step(P, as_stream(stream), params, grads, m1, m2, d_step, targets, out_rgb, lr,
+                       beta1, beta2, eps);
+}
+
+// Captures one forward (kind 0) or one training step (kind 1) with the given
+// arguments into the plan's graph; dsplat_mv_graph_launch replays it.
+extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const float* xyz,
+                                       const float* scale, const float* quat,
+                                       const float* opacity, const float* rgb, uint32_t flags,
+                                       float* out_rgb, float* out_alpha, float* params,
+                                       float* grads, float* m1, float* m2, uint32_t* d_step,
+                                       const float* targets, float lr, float beta1, float beta2,
+                                       float eps) {
+  if (!out_rgb) return dsplat_set_error(DSPLAT_ERR_ARG, "graph: null out_rgb");
+  if (!P || P->nview == 0) return dsplat_set_error(DSPLAT_ERR_ARG, "graph: no views set");
+  if (!P->cap_stream) DSPLAT_HIP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+  if (P->exec) {
+    DSPLAT_HIP(hipGraphExecDestroy(P->exec));
+    P->exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  DSPLAT_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc;
+  if (kind == 0) {
+    rc = (xyz && scale && quat && opacity && rgb)
+             ? mv_forward(P, P->cap_stream, xyz, scale, quat, opacity, rgb, flags, out_rgb,
+                          out_alpha)
+             : dsplat_set_error(DSPLAT_ERR_ARG, "graph: null forward argument");
+  } else if (kind == 2) {
+    const size_t n = P->n;
+    rc = (xyz && scale && quat && opacity && rgb && grads && targets)
+             ? mv_forward(P, P->cap_stream, xyz, scale, quat, opacity, rgb, flags, out_rgb,
+                          out_alpha)
+             : dsplat_set_error(DSPLAT_ERR_ARG, "graph: null forward/backward argument");
+    if (!rc)
+      rc = mv_backward(P, P->cap_stream, xyz, scale, quat, opacity, out_rgb, nullptr, targets,
+                       nullptr, grads, grads + 3 * n, grads + 6 * n, grads + 10 * n,
+                       grads + 11 * n, flags);
+  } else {
+    rc = (params && grads && m1 && m2 && d_step && targets)
+             ? mv_train_step(P, P->cap_stream, params, grads, m1, m2, d_step, targets, out_rgb,
+                             lr, beta1, beta2, eps)
+             : dsplat_set_error(DSPLAT_ERR_ARG, "graph: null training argument");
+  }
+  hipError_t e = hipStreamEndCapture(P->cap_stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return dsplat_set_hip_error(e, "graph: end capture");
+  e = hipGraphInstantiate(&P->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return dsplat_set_hip_error(e, "graph: instantiate");
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_graph_launch(dsplat_mv_plan* P, void* stream) {
+  if (!P || !P->exec) return dsplat_set_error(DSPLAT_ERR_ARG, "graph: nothing captured");
+  DSPLAT_HIP(hipGraphLaunch(P->exec, as_stream(stream)));
+  return DSPLAT_OK;
+}

@@ -1,0 +1,352 @@
+// Per-tile bodies of the Gaussian raster and its adjoint, shared by the
+// single-view kernels (one workgroup per tile) and the persistent multi-view
+// kernels (workgroups pull (view, tile) items from a queue).
+//
+// Forward semantics: rasterize_splat_using_tile (splat_gauss.cu:56-109). For
+// the pixel centre t = (ix+.5, iy+.5) the tile's list is walked from its end
+// (largest depth key = nearest) to its start; a splat is skipped when t lies
+// outside its box; e = exp(-q/2) * opacity with q = d^T C d, d = t - centre;
+// rgb += colour * T e; T *= 1 - e; alpha_sum += T e; the walk stops once
+// alpha_sum > 0.999 (the reference compares the float sum with the double
+// 0.999, which is the same as alpha_sum >= 0.999f: 0.999f is the smallest
+// float above the double 0.999).
+//
+// Records of NT list entries are staged in LDS as float4 quarters (every lane
+// reads the same address: broadcast), every thread walks them for its pixels,
+// and the workgroup leaves the tile as soon as every pixel has terminated
+// (__syncthreads_and).
+//
+// Adjoint: every pixel walks its list back-to-front from the last entry the
+// forward visited (nwalk) with the transmittance recovered by division from
+// tlast (the transmittance in front of the last contributing splat; dividing
+// by 1 - e of earlier splats is safe because the walk did not stop there, so
+// T stayed above ~1e-3). Per list entry the nine gradient terms of all
+// pixels are summed across each 16-lane row with DPP, across rows and waves
+// with LDS float atomics, and once per (tile, entry) into global memory.
+#pragma once
+#include "dsplat_common.hpp"
+
+namespace dsplat {
+
+// Packed records, 4 x float4 per item: (x, y, a, b), (c, opacity, r, g),
+// (b, x0, y0, x1), (y1, -, -, -).
+struct PackedLoader {
+  const float4* rec;
+  __device__ void load(uint32_t p, float4& q0, float4& q1, float4& q2, float4& q3) const {
+    const float4* r = rec + 4 * size_t(p);
+    q0 = r[0];
+    q1 = r[1];
+    q2 = r[2];
+    q3 = r[3];
+  }
+};
+
+// The reference's separate arrays (splat_gauss.cu:61-66).
+struct ArrayLoader {
+  const float* pixxydepth;
+  const float* conic;
+  const float4* aabb;
+  const float* opacity;
+  const float* rgb;
+  __device__ void load(uint32_t p, float4& q0, float4& q1, float4& q2, float4& q3) const {
+    const float4 bb = aabb[p];
+    q0 = make_float4(pixxydepth[3 * p], pixxydepth[3 * p + 1], conic[3 * p], conic[3 * p + 1]);
+    q1 = make_float4(conic[3 * p + 2], opacity[p], rgb[3 * p], rgb[3 * p + 1]);
+    q2 = make_float4(rgb[3 * p + 2], bb.x, bb.y, bb.z);
+    q3 = make_float4(bb.w, 0.f, 0.f, 0.f);
+  }
+};
+
+__device__ inline float fast_exp(float x) { return __expf(x); }
+
+// Forward of one tile. Image arrays are the view's base pointers (pixel
+// index iy * img_w + ix); alpha / nwalk / tlast may be null. The entry loop is
+// outermost so one LDS read of a record serves all PPT pixels of a thread.
+template <int TS, int NT, typename Loader>
+__device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32_t img_w,
+                                                uint32_t img_h, uint32_t start, uint32_t end,
+                                                const Loader& ld,
+                                                const uint32_t* __restrict__ idx2pnt,
+                                                float* __restrict__ pix2rgb,
+                                                float* __restrict__ pix2alpha,
+                                                uint32_t* __restrict__ pix2nwalk,
+                                                float* __restrict__ pix2tlast) {
+  constexpr int PPT = TS * TS / NT;
+  __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
+  const uint32_t num = end - start;
+  float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
+  uint32_t walked[PPT];
+  bool done[PPT];
+  int live = 0;
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t lp = threadIdx.x + k * NT;
+    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    px[k] = float(ix) + 0.5f;
+    py[k] = float(iy) + 0.5f;
+    cr[k] = cg[k] = cbl[k] = asum[k] = 0.f;
+    tr[k] = tl[k] = 1.f;
+    walked[k] = num;
+    done[k] = !(ix < img_w && iy < img_h);
+    live += done[k] ? 0 : 1;
+  }
+  for (uint32_t b = 0; b < num; b += NT) {
+    const uint32_t it = b + threadIdx.x;
+    if (it < num) {
+      float4 q0, q1, q2, q3;
+      ld.load(idx2pnt[end - 1 - it], q0, q1, q2, q3);
+      s0[threadIdx.x] = q0;
+      s1[threadIdx.x] = q1;
+      s2[threadIdx.x] = q2;
+      s3[threadIdx.x] = q3;
+    }
+    __syncthreads();
+    const uint32_t cnt = min(uint32_t(NT), num - b);
+    for (uint32_t j = 0; j < cnt && live > 0; ++j) {
+      const float4 q2 = s2[j];
+      const float4 q3 = s3[j];
+      bool in[PPT];
+      bool anyin = false;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        // inclusive box test (aabb2::is_include_point)
+        in[k] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
+        anyin = anyin || in[k];
+      }
+      if (!anyin) continue;
+      const float4 q0 = s0[j];
+      const float4 q1 = s1[j];
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        if (!in[k]) continue;
+        const float dx = px[k] - q0.x, dy = py[k] - q0.y;
+        const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
+        const float e = fast_exp(-0.5f * q) * q1.y;
+        const float eo = tr[k] * e;
+        cr[k] += q1.z * eo;
+        cg[k] += q1.w * eo;
+        cbl[k] += q2.x * eo;
+        tl[k] = tr[k];
+        tr[k] *= 1.f - e;
+        asum[k] += eo;
+        if (asum[k] >= 0.999f) {
+          done[k] = true;
+          walked[k] = b + j + 1;
+          --live;
+        }
+      }
+    }
+    if (__syncthreads_and(live == 0)) break;
+  }
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t lp = threadIdx.x + k * NT;
+    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    if (ix >= img_w || iy >= img_h) continue;
+    const size_t ip = size_t(iy) * img_w + ix;
+    pix2rgb[3 * ip + 0] = cr[k];
+    pix2rgb[3 * ip + 1] = cg[k];
+    pix2rgb[3 * ip + 2] = cbl[k];
+    if (pix2alpha) pix2alpha[ip] = asum[k];
+    if (pix2nwalk) pix2nwalk[ip] = walked[k];
+    if (pix2tlast) pix2tlast[ip] = tl[k];
+  }
+}
+
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over each 16-lane row; every lane of the row receives the row sum.
+__device__ inline float row_sum16(float v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  return v;
+}
+
+constexpr int kGrad = 9;  // dx, dy, dca, dcb, dcc, dop, dr, dg, db
+
+// Gradient sinks of the adjoint: 2D terms per list item, opacity / colour per
+// Gaussian (item - gauss_base; 0 for a single view).
+struct GradSink {
+  float* d_pixxy;   // [items, 2]
+  float* d_conic;   // [items, 3]
+  float* d_op;      // [gaussians]
+  float* d_rgb;     // [gaussians, 3]
+};
+
+// Adjoint of one tile. dL/drgb is read from d_img, or -- when d_img is null
+// -- formed in place as 2 (rgb - target) for the loss sum (rgb - target)^2.
+// kAlpha: whether dL/dalpha is present (d_alpha != null).
+template <int TS, int NT, bool kAlpha>
+__device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, uint32_t img_w,
+                                                uint32_t img_h, uint32_t end,
+                                                const float4* __restrict__ splat2,
+                                                const uint32_t* __restrict__ idx2pnt,
+                                                const uint32_t* __restrict__ pix2nwalk,
+                                                const float* __restrict__ pix2tlast,
+                                                const float* __restrict__ d_img,
+                                                const float* __restrict__ pix2rgb,
+                                                const float* __restrict__ target,
+                                                const float* __restrict__ d_alpha,
+                                                uint32_t gauss_base, const GradSink& out) {
+  constexpr int PPT = TS * TS / NT;
+  __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
+  __shared__ uint32_t spid[NT];
+  __shared__ float acc[kGrad][NT];
+  __shared__ uint32_t smax;
+  const int lane = threadIdx.x & 63;
+  float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
+  float b0[PPT], b1[PPT], b2[PPT], ba[PPT];
+  uint32_t nw[PPT];
+  bool first[PPT];
+  uint32_t my_max = 0;
+#pragma unroll
+  for (int k = 0; k < PPT; ++k) {
+    const uint32_t lp = threadIdx.x + k * NT;
+    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    px[k] = float(ix) + 0.5f;
+    py[k] = float(iy) + 0.5f;
+    first[k] = true;
+    b0[k] = b1[k] = b2[k] = ba[k] = 0.f;
+    if (ix < img_w && iy < img_h) {
+      const size_t ip = size_t(iy) * img_w + ix;
+      nw[k] = pix2nwalk[ip];
+      tr[k] = pix2tlast[ip];
+      if (d_img) {
+        g0[k] = d_img[3 * ip];
+        g1[k] = d_img[3 * ip + 1];
+        g2[k] = d_img[3 * ip + 2];
+      } else {
+        g0[k] = 2.f * (pix2rgb[3 * ip] - target[3 * ip]);
+        g1[k] = 2.f * (pix2rgb[3 * ip + 1] - target[3 * ip + 1]);
+        g2[k] = 2.f * (pix2rgb[3 * ip + 2] - target[3 * ip + 2]);
+      }
+      ga[k] = d_alpha ? d_alpha[ip] : 0.f;
+    } else {
+      nw[k] = 0;
+      tr[k] = 1.f;
+      g0[k] = g1[k] = g2[k] = ga[k] = 0.f;
+    }
+    my_max = max(my_max, nw[k]);
+  }
+  if (threadIdx.x == 0) smax = 0;
+  __syncthreads();
+  atomicMax(&smax, my_max);
+  __syncthreads();
+  const uint32_t maxw = smax;
+  for (uint32_t hi = maxw; hi > 0;) {
+    const uint32_t lo = hi > uint32_t(NT) ? hi - NT : 0u;
+    const uint32_t cnt = hi - lo;
+    if (threadIdx.x < cnt) {
+      const uint32_t it = hi - 1 - threadIdx.x;
+      const uint32_t p = idx2pnt[end - 1 - it];
+      const float4* r = splat2 + 4 * size_t(p);
+      s0[threadIdx.x] = r[0];
+      s1[threadIdx.x] = r[1];
+      s2[threadIdx.x] = r[2];
+      s3[threadIdx.x] = r[3];
+      spid[threadIdx.x] = p;
+#pragma unroll
+      for (int c = 0; c < kGrad; ++c) acc[c][threadIdx.x] = 0.f;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t it = hi - 1 - j;
+      const float4 q0 = s0[j], q1 = s1[j], q2 = s2[j], q3 = s3[j];
+      float v[kGrad];
+#pragma unroll
+      for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const bool in = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
+                        py[k] <= q3.x;
+        if (!in) continue;
+        any = true;
+        const float dx = px[k] - q0.x, dy = py[k] - q0.y;
+        const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
+        const float gk = fast_exp(-0.5f * q);
+        const float e = gk * q1.y;
+        const float ome = 1.f - e;
+        const float inv = __builtin_amdgcn_rcpf(ome);
+        if (!first[k]) tr[k] = tr[k] * inv;
+        first[k] = false;
+        const float t = tr[k];
+        float dl_de = g0[k] * (q1.z * t - b0[k] * inv) + g1[k] * (q1.w * t - b1[k] * inv) +
+                      g2[k] * (q2.x * t - b2[k] * inv);
+        if (kAlpha) dl_de += ga[k] * (t - ba[k] * inv);
+        const float et = e * t;
+        v[6] += g0[k] * et;
+        v[7] += g1[k] * et;
+        v[8] += g2[k] * et;
+        b0[k] += q1.z * et;
+        b1[k] += q1.w * et;
+        b2[k] += q2.x * et;
+        if (kAlpha) ba[k] += et;
+        v[5] += dl_de * gk;
+        const float dl_dq = dl_de * e * -0.5f;
+        v[2] += dl_dq * dx * dx;
+        v[3] += dl_dq * 2.f * dx * dy;
+        v[4] += dl_dq * dy * dy;
+        v[0] -= dl_dq * (2.f * q0.z * dx + 2.f * q0.w * dy);
+        v[1] -= dl_dq * (2.f * q0.w * dx + 2.f * q1.x * dy);
+      }
+      if (__ballot(any) == 0) continue;  // wave-uniform
+#pragma unroll
+      for (int c = 0; c < kGrad; ++c) {
+        const float r = row_sum16(v[c]);
+        if ((lane & 15) == 0) atomicAdd(&acc[c][j], r);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+      const uint32_t p = spid[threadIdx.x];
+      const uint32_t g = p - gauss_base;
+      const float a0 = acc[0][threadIdx.x], a1 = acc[1][threadIdx.x];
+      if (a0 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p)], a0);
+      if (a1 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p) + 1], a1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float a = acc[2 + c][threadIdx.x];
+        if (a != 0.f) atomicAdd(&out.d_conic[3 * size_t(p) + c], a);
+      }
+      const float ao = acc[5][threadIdx.x];
+      if (ao != 0.f) atomicAdd(&out.d_op[g], ao);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float a = acc[6 + c][threadIdx.x];
+        if (a != 0.f) atomicAdd(&out.d_rgb[3 * size_t(g) + c], a);
+      }
+    }
+    __syncthreads();
+    hi = lo;
+  }
+}
+
+template <int TS, int NT>
+__device__ __forceinline__ void raster_tile_bwd(uint32_t tx, uint32_t ty, uint32_t img_w,
+                                                uint32_t img_h, uint32_t end,
+                                                const float4* __restrict__ splat2,
+                                                const uint32_t* __restrict__ idx2pnt,
+                                                const uint32_t* __restrict__ pix2nwalk,
+                                                const float* __restrict__ pix2tlast,
+                                                const float* __restrict__ d_img,
+                                                const float* __restrict__ pix2rgb,
+                                                const float* __restrict__ target,
+                                                const float* __restrict__ d_alpha,
+                                                uint32_t gauss_base, const GradSink& out) {
+  if (d_alpha)
+    raster_tile_bwd_impl<TS, NT, true>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
+                                       pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
+                                       out);
+  else
+    raster_tile_bwd_impl<TS, NT, false>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
+                                        pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
+                                        out);
+}
+
+}  // namespace dsplat

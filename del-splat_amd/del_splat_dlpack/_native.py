@@ -16,8 +16,16 @@ _u32 = ctypes.c_uint32
 _f32 = ctypes.c_float
 _sz = ctypes.c_size_t
 _int = ctypes.c_int
+_u64 = ctypes.c_uint64
 
 DSPLAT_SCALE_IS_LOG = 1
+DSPLAT_OPACITY_IS_LOGIT = 2
+
+
+class ViewDesc(ctypes.Structure):
+    """dsplat_view_desc: column-major world->ndc 4x4 and the image size."""
+    _fields_ = [("world2ndc", ctypes.c_float * 16), ("img_w", ctypes.c_uint32),
+                ("img_h", ctypes.c_uint32)]
 
 # name -> (restype, argtypes)
 _SIGS = {
@@ -41,6 +49,18 @@ _SIGS = {
     "dsplat_point_rasterize_zbuffer": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "dsplat_circle_rasterize_zbuffer": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "dsplat_point_rasterize_pix": (_int, [_vp, _u32, _vp, _vp, _vp, _u32, _u32, _vp]),
+    "dsplat_mv_plan_create": (_int, [_u32, _u32, _u64, _u32, _u32, _u32, ctypes.POINTER(_vp)]),
+    "dsplat_mv_plan_destroy": (_int, [_vp]),
+    "dsplat_mv_set_views": (_int, [_vp, _vp, _u32, _vp]),
+    "dsplat_mv_plan_info": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "dsplat_mv_outputs": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "dsplat_mv_forward": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
+    "dsplat_mv_backward": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32]),
+    "dsplat_adam_step": (_int, [_vp, _u64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _vp]),
+    "dsplat_mv_train_step": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32]),
+    "dsplat_mv_graph_capture": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32]),
+    "dsplat_mv_graph_launch": (_int, [_vp, _vp]),
+    "dsplat_memcpy_d2d": (_int, [_vp, _vp, _vp, _u64]),
     "dsplat_exclusive_scan_workspace_size": (_sz, [_u32]),
     "dsplat_exclusive_scan_u32": (_int, [_vp, _u32, _vp, _vp, _vp, _sz]),
     "dsplat_radix_sort_workspace_size": (_sz, [_u32]),

@@ -230,6 +230,96 @@ int dsplat_point_rasterize_pix(void* stream, uint32_t num_pnt, const float* pnt2
                                uint32_t img_h, uint64_t* pix2key);
 
 /* ======================================================================
+ * Batched multi-view rendering / differentiation and Adam (no counterpart in
+ * the reference, which renders one view per call)
+ * ====================================================================== */
+
+#define DSPLAT_OPACITY_IS_LOGIT 2u  /* opacity input holds logit(opacity) */
+
+/* One camera: column-major world->ndc 4x4 (host memory) and image size. */
+typedef struct dsplat_view_desc {
+  float world2ndc[16];
+  uint32_t img_w, img_h;
+} dsplat_view_desc;
+
+/* Plan for N Gaussians and up to max_view views whose images total at most
+ * max_pixels pixels, with tile size ts, raster workgroup size `threads` (as
+ * dsplat_gauss_rasterize_fwd) and room for pair_capacity (tile, item) pairs.
+ * Allocates all device memory once (hipMalloc); frames then allocate
+ * nothing and never synchronise with the host. */
+typedef struct dsplat_mv_plan dsplat_mv_plan;
+int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64_t max_pixels,
+                          uint32_t tile_size, uint32_t threads, uint32_t pair_capacity,
+                          dsplat_mv_plan** plan);
+int dsplat_mv_plan_destroy(dsplat_mv_plan* plan);
+
+/* SYNC. Sets the views of the next frames (host array of num_view). Images
+ * are concatenated in view order: view v's pixels start at pix_offsets[v]. */
+int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
+                        const dsplat_view_desc* views);
+
+/* Host query: view count, total pixels/tiles and the per-view pixel and
+ * global-tile offsets (arrays of num_view+1, may be null). */
+int dsplat_mv_plan_info(const dsplat_mv_plan* plan, uint32_t* num_view, uint64_t* total_pix,
+                        uint32_t* total_tiles, uint64_t* pix_offsets, uint32_t* tile_offsets);
+
+/* Device pointers of plan-owned state: alpha [pix], nwalk u32 [pix], tlast
+ * [pix], tile2idx u32 [tiles+1] (global tile ids), idx2pnt u32 [capacity]
+ * (item = view*N + Gaussian), counters u32 [5] = (K, K if it fit else 0, -, -,
+ * overflow flag). Any argument may be null. rgb is reserved (null). */
+int dsplat_mv_outputs(const dsplat_mv_plan* plan, float** rgb, float** alpha, uint32_t** nwalk,
+                      float** tlast, uint32_t** tile2idx, uint32_t** idx2pnt,
+                      uint32_t** counters);
+
+/* Forward of all views: projection, binning + stable sort, persistent tiled
+ * raster into out_rgb [pix,3] and out_alpha [pix] (null: plan-owned). flags:
+ * DSPLAT_SCALE_IS_LOG, DSPLAT_OPACITY_IS_LOGIT. */
+int dsplat_mv_forward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
+                      const float* pnt2scale, const float* pnt2quat, const float* pnt2opacity,
+                      const float* pnt2rgb, uint32_t flags, float* out_rgb, float* out_alpha);
+
+/* Adjoint of the last forward, summed over views. dL/drgb comes from d_img
+ * [pix,3] or, when d_img is null, from the loss sum (out_rgb - target)^2 with
+ * target [pix,3]. d_alpha [pix] may be null. Overwrites d_pnt2xyz [N,3],
+ * d_pnt2scale [N,3], d_pnt2quat [N,4], d_pnt2opacity [N], d_pnt2rgb [N,3]. */
+int dsplat_mv_backward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
+                       const float* pnt2scale, const float* pnt2quat, const float* pnt2opacity,
+                       const float* out_rgb, const float* d_img, const float* target,
+                       const float* d_alpha, float* d_pnt2xyz, float* d_pnt2scale,
+                       float* d_pnt2quat, float* d_pnt2opacity, float* d_pnt2rgb,
+                       uint32_t flags);
+
+/* Adam (Kingma & Ba 2015) over n values; *d_step (device u32) is incremented
+ * first and gives the bias correction, so a captured graph can replay it. */
+int dsplat_adam_step(void* stream, uint64_t n, float* param, const float* grad, float* m1,
+                     float* m2, float lr, float beta1, float beta2, float eps, uint32_t* d_step);
+
+/* One training step on the plan's views over the flat parameter block
+ * params [14N] = [xyz 3N | log-scale 3N | quat 4N | opacity logit N | rgb 3N]
+ * (N even): forward into out_rgb, adjoint of sum (out_rgb - targets)^2 into
+ * grads [14N], Adam on all 14N values (moments m1, m2 [14N]). */
+int dsplat_mv_train_step(dsplat_mv_plan* plan, void* stream, float* params, float* grads,
+                         float* m1, float* m2, uint32_t* d_step, const float* targets,
+                         float* out_rgb, float lr, float beta1, float beta2, float eps);
+
+/* Captures one forward (kind 0: the dsplat_mv_forward arguments), one
+ * training step (kind 1: the dsplat_mv_train_step arguments) or one forward
+ * + L2 adjoint (kind 2: forward arguments, targets, and grads [14N] laid out
+ * as the training parameter block) into the plan's hipGraph;
+ * dsplat_mv_graph_launch replays it on any stream. The pointers are baked
+ * into the graph. */
+int dsplat_mv_graph_capture(dsplat_mv_plan* plan, uint32_t kind, const float* pnt2xyz,
+                            const float* pnt2scale, const float* pnt2quat,
+                            const float* pnt2opacity, const float* pnt2rgb, uint32_t flags,
+                            float* out_rgb, float* out_alpha, float* params, float* grads,
+                            float* m1, float* m2, uint32_t* d_step, const float* targets,
+                            float lr, float beta1, float beta2, float eps);
+int dsplat_mv_graph_launch(dsplat_mv_plan* plan, void* stream);
+
+/* Stream-ordered device-to-device copy of `bytes` bytes. */
+int dsplat_memcpy_d2d(void* stream, void* dst, const void* src, uint64_t bytes);
+
+/* ======================================================================
  * GPU primitives (replacing del-cudarc-sys)
  * ====================================================================== */
 
