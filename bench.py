@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--num-gauss", type=int, default=1_000_000)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=0, help="0: autotune tile size and threads")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -79,6 +79,14 @@ def headline(args, torch, np, SG, scene):
     xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
     m16 = t(m)
     target = t(scene.target_image(shape, 102))
+    tune = None
+    if ts == 0:
+        from del_splat_dlpack import autotune
+        (ts, args.threads), tune = autotune.tune_single(args.num_gauss, m16, shape,
+                                                        (xyz, ls, q, op, rgb), target,
+                                                        use_cache=False)
+        args.tile = ts
+        args.autotune = tune
     stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
     ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for k in stages}
@@ -142,15 +150,18 @@ def batched(args, torch, np, SG, scene):
         tflat = torch.cat([x.reshape(-1) for x in targets])
         mats = [t(mm) for mm, _ in views]
 
+        # the per-view path runs with the single-view autotuned schedule
+        pts, pth = (args.tile or 16), args.threads
+
         def per_view():
             for v, (_, shape) in enumerate(views):
                 pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, mats[v], shape)
-                t2i, i2p = SG.bin_gauss(shape, 16, pxd, aabb)
-                img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, 16, splat2, t2i, i2p)
+                t2i, i2p = SG.bin_gauss(shape, pts, pxd, aabb)
+                img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, pts, splat2, t2i, i2p, pth)
                 if mode == "fwd_bwd":
                     d_img = 2.0 * (img - targets[v])
-                    dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, 16, splat2, t2i, i2p, nwalk,
-                                                             tlast, d_img)
+                    dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, pts, splat2, t2i, i2p, nwalk,
+                                                             tlast, d_img, None, pth)
                     SG.project_bwd(xyz, ls, q, mats[v], shape, dpix, dcon)
 
         per_view()
@@ -242,8 +253,10 @@ def main():
         "data": "synthetic (seeded Gaussians and target image)",
         "config": {"workload": f"3dgs fwd+bwd, {N} Gaussians, {shape[0]}x{shape[1]}, tile "
                                f"{args.tile}", "num_gauss": N, "width": shape[0],
-                   "height": shape[1], "tile": args.tile, "pairs_K": K,
-                   "parallelism": "single GPU"},
+                   "height": shape[1], "tile": args.tile, "threads": args.threads,
+                   "pairs_K": K, "parallelism": "single GPU",
+                   "autotune_ms": {k: round(v, 3) for k, v in
+                                   (getattr(args, "autotune", None) or {}).items()}},
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -72,20 +72,19 @@ bin_fill_kernel(uint32_t n, Boxes boxes, const uint32_t* __restrict__ order,
                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ n_eff,
                 uint32_t ts, uint32_t tw, uint32_t th, uint32_t* __restrict__ tile_of_pair,
                 uint32_t* __restrict__ pnt_of_pair) {
+  __shared__ ExpandLds lds[4];
+  if (*n_eff == 0) return;  // uniform: overflow, nothing is written
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  if (*n_eff == 0) return;
-  const uint32_t g = order[j];
-  const float4 b = boxes(g);
-  const TileRect r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
-  uint32_t o = off[j];
-  for (int ix = r.x0; ix < r.x1; ++ix) {
-    for (int iy = r.y0; iy < r.y1; ++iy) {
-      tile_of_pair[o] = uint32_t(iy) * tw + uint32_t(ix);
-      pnt_of_pair[o] = g;
-      ++o;
-    }
+  const bool valid = j < n;
+  uint32_t g = 0, o = 0;
+  TileRect r{0, 0, 0, 0};
+  if (valid) {
+    g = order[j];
+    const float4 b = boxes(g);
+    r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
+    o = off[j];
   }
+  wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair);
 }
 
 // tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T].

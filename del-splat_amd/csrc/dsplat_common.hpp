@@ -76,6 +76,54 @@ __device__ inline TileRect tile_rect(float x0, float y0, float x1, float y1, uin
   return r;
 }
 
+// Per-wave LDS slice of wave_expand_pairs.
+struct ExpandLds {
+  uint32_t off[64], x0[64], y0[64], h[64], tbase[64], tw[64], val[64];
+};
+
+// Writes the (tile, value) pairs of the calling wave's 64 consecutive items:
+// item l owns output slots [off, off + count) and covers the tile rectangle
+// r (ix-outer, iy-inner, tile id = tbase + iy * tw + ix). Items are valid on
+// a prefix of the lanes with non-decreasing offsets. Every lane writes
+// consecutive slots of the wave's range, so the stores are coalesced; the
+// item of a slot is found by binary search over the wave's offsets.
+__device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, uint32_t tbase,
+                                         uint32_t tw, uint32_t val, ExpandLds& L,
+                                         uint32_t* __restrict__ tile_out,
+                                         uint32_t* __restrict__ val_out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t vmask = __ballot(valid);
+  if (vmask == 0) return;
+  const int nvalid = __popcll(vmask);
+  const uint32_t cnt = valid ? uint32_t(r.count()) : 0u;
+  L.off[lane] = valid ? off : 0xffffffffu;
+  L.x0[lane] = uint32_t(r.x0);
+  L.y0[lane] = uint32_t(r.y0);
+  L.h[lane] = uint32_t(r.y1 - r.y0);
+  L.tbase[lane] = tbase;
+  L.tw[lane] = tw;
+  L.val[lane] = val;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const uint32_t base = __builtin_amdgcn_readfirstlane(off);
+  const uint32_t end = __builtin_amdgcn_readlane(off + cnt, nvalid - 1);
+  for (uint32_t e = base + lane; e < end; e += 64) {
+    int lo = 0, hi = nvalid - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (L.off[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t c = e - L.off[lo];
+    const uint32_t h = L.h[lo];
+    const uint32_t ix = L.x0[lo] + c / h, iy = L.y0[lo] + c % h;
+    tile_out[e] = L.tbase[lo] + iy * L.tw[lo] + ix;
+    val_out[e] = L.val[lo];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Packed 2D splat record used by the raster kernels (one 64-byte line per
 // Gaussian): pixel centre, conic [a b c], opacity, rgb, pixel box.
 struct __align__(16) Splat2 {

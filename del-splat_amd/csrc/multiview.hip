@@ -107,22 +107,24 @@ mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views, u
                const uint32_t* __restrict__ order, const uint32_t* __restrict__ off,
                const uint32_t* __restrict__ misc, const float4* __restrict__ rec,
                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ item_of_pair) {
+  __shared__ ExpandLds lds[4];
+  if (misc[1] == 0) return;  // uniform: overflow, nothing is written
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nitems) return;
-  if (misc[1] == 0) return;
-  const uint32_t item = order[j];
-  const uint32_t v = item / n;
-  const ViewDev& vd = views[v];
-  const float4 q2 = rec[4 * size_t(item) + 2];
-  const float4 q3 = rec[4 * size_t(item) + 3];
-  const TileRect r = tile_rect(q2.y, q2.z, q2.w, q3.x, ts, vd.tw, vd.th);
-  uint32_t o = off[j];
-  for (int ix = r.x0; ix < r.x1; ++ix)
-    for (int iy = r.y0; iy < r.y1; ++iy) {
-      tile_of_pair[o] = vd.tile_off + uint32_t(iy) * vd.tw + uint32_t(ix);
-      item_of_pair[o] = item;
-      ++o;
-    }
+  const bool valid = j < nitems;
+  uint32_t item = 0, o = 0, tbase = 0, tw = 1;
+  TileRect r{0, 0, 0, 0};
+  if (valid) {
+    item = order[j];
+    const ViewDev& vd = views[item / n];
+    const float4 q2 = rec[4 * size_t(item) + 2];
+    const float4 q3 = rec[4 * size_t(item) + 3];
+    r = tile_rect(q2.y, q2.z, q2.w, q3.x, ts, vd.tw, vd.th);
+    o = off[j];
+    tbase = vd.tile_off;
+    tw = vd.tw;
+  }
+  wave_expand_pairs(valid, o, r, tbase, tw, item, lds[threadIdx.x >> 6], tile_of_pair,
+                    item_of_pair);
 }
 
 __global__ void __launch_bounds__(256)

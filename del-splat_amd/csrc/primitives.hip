@@ -167,7 +167,11 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
                      const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
                      uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
                      uint32_t nblocks, const uint32_t* __restrict__ hist) {
+  static_assert(kSortBlock == 256, "one thread per digit");
   __shared__ uint32_t wcnt[kSortWaves][256];
+  __shared__ uint32_t dpre[256];
+  __shared__ KeyT skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
   const uint32_t n = load_n(n_cap, d_n);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
@@ -197,24 +201,40 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     if (valid && below == 0) wcnt[w][d] = running + uint32_t(__popcll(peers));
   }
   __syncthreads();
-  // exclusive prefix of the per-wave counts over waves, per digit
-  if (threadIdx.x < 256) {
-    uint32_t run = 0;
+  // exclusive prefix of the per-wave counts over waves, per digit, and the
+  // block's digit totals
+  uint32_t total = 0;
+  {
 #pragma unroll
     for (int k = 0; k < kSortWaves; ++k) {
       const uint32_t c = wcnt[k][threadIdx.x];
-      wcnt[k][threadIdx.x] = run;
-      run += c;
+      wcnt[k][threadIdx.x] = total;
+      total += c;
     }
   }
+  // dpre[d]: elements of the block with a smaller digit
+  uint32_t btot;
+  dpre[threadIdx.x] = block_exclusive_scan(total, &btot);
   __syncthreads();
+  // stage in LDS in block-sorted order ...
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     if (dig[r] == 0xffffffffu) continue;
     const uint32_t d = dig[r];
-    const uint32_t pos = hist[d * nblocks + blockIdx.x] + wcnt[w][d] + rank[r];
-    kout[pos] = key[r];
-    vout[pos] = val[r];
+    const uint32_t lp = dpre[d] + wcnt[w][d] + rank[r];
+    skey[lp] = key[r];
+    sval[lp] = val[r];
+  }
+  __syncthreads();
+  // ... and write each digit's run to consecutive global addresses
+  const uint64_t bstart = uint64_t(blockIdx.x) * kSortTile;
+  const uint32_t nb_valid = bstart >= n ? 0u : uint32_t(min(uint64_t(kSortTile), n - bstart));
+  for (uint32_t i = threadIdx.x; i < nb_valid; i += kSortBlock) {
+    const KeyT k = skey[i];
+    const uint32_t d = uint32_t(k >> shift) & mask;
+    const uint32_t pos = hist[d * nblocks + blockIdx.x] + (i - dpre[d]);
+    kout[pos] = k;
+    vout[pos] = sval[i];
   }
 }
 

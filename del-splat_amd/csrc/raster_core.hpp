@@ -167,7 +167,25 @@ __device__ inline float row_sum16(float v) {
   return v;
 }
 
-constexpr int kGrad = 9;  // dx, dy, dca, dcb, dcc, dop, dr, dg, db
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_rows(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the 64 lanes of a wave; lane 63 receives the total. Row sums by
+// quad/mirror DPP, then row_bcast:15 adds row 0 into row 1 and row 2 into
+// row 3, and row_bcast:31 adds lane 31 into rows 2 and 3.
+__device__ inline float wave_sum_to_last(float v) {
+  v = row_sum16(v);
+  v += dpp_rows<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
+  v += dpp_rows<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
+  return v;
+}
+
+// Reduced per list entry: S_x, S_y, S_xx, S_xy, S_yy (moments of dL/dq over
+// the pixel offsets), dL/dopacity, dL/drgb.
+constexpr int kGrad = 9;
 
 // Gradient sinks of the adjoint: 2D terms per list item, opacity / colour per
 // Gaussian (item - gauss_base; 0 for a single view).
@@ -181,6 +199,16 @@ struct GradSink {
 // Adjoint of one tile. dL/drgb is read from d_img, or -- when d_img is null
 // -- formed in place as 2 (rgb - target) for the loss sum (rgb - target)^2.
 // kAlpha: whether dL/dalpha is present (d_alpha != null).
+// LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
+// template instance declaring its own arrays would double the footprint).
+template <int NT>
+struct BwdLds {
+  float4 s0[NT], s1[NT], s2[NT], s3[NT];
+  uint32_t spid[NT];
+  float acc[kGrad][NT];
+  uint32_t smax;
+};
+
 template <int TS, int NT, bool kAlpha>
 __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, uint32_t img_w,
                                                 uint32_t img_h, uint32_t end,
@@ -192,12 +220,16 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float* __restrict__ pix2rgb,
                                                 const float* __restrict__ target,
                                                 const float* __restrict__ d_alpha,
-                                                uint32_t gauss_base, const GradSink& out) {
+                                                uint32_t gauss_base, const GradSink& out,
+                                                BwdLds<NT>& L) {
   constexpr int PPT = TS * TS / NT;
-  __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
-  __shared__ uint32_t spid[NT];
-  __shared__ float acc[kGrad][NT];
-  __shared__ uint32_t smax;
+  float4* s0 = L.s0;
+  float4* s1 = L.s1;
+  float4* s2 = L.s2;
+  float4* s3 = L.s3;
+  uint32_t* spid = L.spid;
+  auto& acc = L.acc;
+  uint32_t& smax = L.smax;
   const int lane = threadIdx.x & 63;
   float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
   float b0[PPT], b1[PPT], b2[PPT], ba[PPT];
@@ -288,31 +320,41 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         b2[k] += q2.x * et;
         if (kAlpha) ba[k] += et;
         v[5] += dl_de * gk;
+        // moments of dL/dq: the mean and conic gradients of the entry are
+        // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
         const float dl_dq = dl_de * e * -0.5f;
-        v[2] += dl_dq * dx * dx;
-        v[3] += dl_dq * 2.f * dx * dy;
-        v[4] += dl_dq * dy * dy;
-        v[0] -= dl_dq * (2.f * q0.z * dx + 2.f * q0.w * dy);
-        v[1] -= dl_dq * (2.f * q0.w * dx + 2.f * q1.x * dy);
+        const float a = dl_dq * dx, bq = dl_dq * dy;
+        v[0] += a;
+        v[1] += bq;
+        v[2] += a * dx;
+        v[3] += a * dy;
+        v[4] += bq * dy;
       }
       if (__ballot(any) == 0) continue;  // wave-uniform
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) {
-        const float r = row_sum16(v[c]);
-        if ((lane & 15) == 0) atomicAdd(&acc[c][j], r);
+        const float r = wave_sum_to_last(v[c]);
+        if (lane == 63) {
+          if (NT == 64) acc[c][j] = r;
+          else atomicAdd(&acc[c][j], r);
+        }
       }
     }
     __syncthreads();
     if (threadIdx.x < cnt) {
       const uint32_t p = spid[threadIdx.x];
       const uint32_t g = p - gauss_base;
-      const float a0 = acc[0][threadIdx.x], a1 = acc[1][threadIdx.x];
+      const float sx = acc[0][threadIdx.x], sy = acc[1][threadIdx.x];
+      const float ca = s0[threadIdx.x].z, cb = s0[threadIdx.x].w, cc = s1[threadIdx.x].x;
+      // d q / d mean = -2 C d  ->  dL/dmean = -2 C (S_x, S_y)
+      const float a0 = -2.f * (ca * sx + cb * sy), a1 = -2.f * (cb * sx + cc * sy);
       if (a0 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p)], a0);
       if (a1 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p) + 1], a1);
+      // d q / d(a, b, c) = (dx^2, 2 dx dy, dy^2)
+      const float dc[3] = {acc[2][threadIdx.x], 2.f * acc[3][threadIdx.x], acc[4][threadIdx.x]};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const float a = acc[2 + c][threadIdx.x];
-        if (a != 0.f) atomicAdd(&out.d_conic[3 * size_t(p) + c], a);
+        if (dc[c] != 0.f) atomicAdd(&out.d_conic[3 * size_t(p) + c], dc[c]);
       }
       const float ao = acc[5][threadIdx.x];
       if (ao != 0.f) atomicAdd(&out.d_op[g], ao);
@@ -339,14 +381,15 @@ __device__ __forceinline__ void raster_tile_bwd(uint32_t tx, uint32_t ty, uint32
                                                 const float* __restrict__ target,
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out) {
+  __shared__ BwdLds<NT> lds;
   if (d_alpha)
     raster_tile_bwd_impl<TS, NT, true>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
                                        pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
-                                       out);
+                                       out, lds);
   else
     raster_tile_bwd_impl<TS, NT, false>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
                                         pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
-                                        out);
+                                        out, lds);
 }
 
 }  // namespace dsplat

@@ -35,6 +35,43 @@ def _store(db):
         pass
 
 
+def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=None,
+                use_cache=True):
+    """Single-view forward + L2 adjoint (the differentiable render path).
+    Returns ((tile_size, threads), {config: ms})."""
+    from .SplatGauss import torch as SG
+
+    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd")
+    db = _load() if use_cache else {}
+    if sig in db:
+        e = db[sig]
+        return (e["best"][0], e["best"][1]), dict(e["ms"])
+    xyz, ls, q, op, rgb = tensors
+    res = {}
+    for ts, th in candidates or CANDIDATES_BWD:
+        def step():
+            pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, img_shape)
+            t2i, i2p = SG.bin_gauss(img_shape, ts, pxd, aabb)
+            img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th)
+            d = SG.rasterize_bwd(img_shape, ts, splat2, t2i, i2p, nwalk, tlast,
+                                 2.0 * (img - target), None, th)
+            SG.project_bwd(xyz, ls, q, m16, img_shape, d[0], d[1])
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        res[f"{ts}/{th}"] = (time.perf_counter() - t0) * 1000.0 / reps
+    best = min(res, key=res.get)
+    ts, th = (int(x) for x in best.split("/"))
+    db[sig] = {"best": [ts, th], "ms": res}
+    if use_cache:
+        _store(db)
+    return (ts, th), res
+
+
 def tune_multiview(num_pnt, views, tensors, mode="forward", target=None, reps=3,
                    candidates=None, use_cache=True):
     """Returns ((tile_size, threads), {config: ms}). `tensors` are (xyz,
