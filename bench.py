@@ -43,7 +43,21 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run 5 device-to-device copies of MB megabytes (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mb):
+    import torch
+    n = mb * (1 << 20) // 4
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
 def cpu_baseline(s, m, shape, ts):
@@ -80,11 +94,11 @@ def headline(args, torch, np, SG, scene):
     m16 = t(m)
     target = t(scene.target_image(shape, 102))
     tune = None
+    args.threads_bwd = args.threads
     if ts == 0:
         from del_splat_dlpack import autotune
-        (ts, args.threads), tune = autotune.tune_single(args.num_gauss, m16, shape,
-                                                        (xyz, ls, q, op, rgb), target,
-                                                        use_cache=False)
+        (ts, args.threads, args.threads_bwd), tune = autotune.tune_single(
+            args.num_gauss, m16, shape, (xyz, ls, q, op, rgb), target, use_cache=False)
         args.tile = ts
         args.autotune = tune
     stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
@@ -110,7 +124,7 @@ def headline(args, torch, np, SG, scene):
         mark("loss_grad", False)
         mark("raster_bwd", True)
         dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast,
-                                                 d_img, None, args.threads)
+                                                 d_img, None, args.threads_bwd)
         mark("raster_bwd", False)
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
@@ -151,7 +165,7 @@ def batched(args, torch, np, SG, scene):
         mats = [t(mm) for mm, _ in views]
 
         # the per-view path runs with the single-view autotuned schedule
-        pts, pth = (args.tile or 16), args.threads
+        pts, pth, pthb = (args.tile or 16), args.threads, getattr(args, "threads_bwd", 0)
 
         def per_view():
             for v, (_, shape) in enumerate(views):
@@ -161,7 +175,7 @@ def batched(args, torch, np, SG, scene):
                 if mode == "fwd_bwd":
                     d_img = 2.0 * (img - targets[v])
                     dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, pts, splat2, t2i, i2p, nwalk,
-                                                             tlast, d_img, None, pth)
+                                                             tlast, d_img, None, pthb)
                     SG.project_bwd(xyz, ls, q, mats[v], shape, dpix, dcon)
 
         per_view()
@@ -210,6 +224,9 @@ def batched(args, torch, np, SG, scene):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
@@ -253,7 +270,8 @@ def main():
         "data": "synthetic (seeded Gaussians and target image)",
         "config": {"workload": f"3dgs fwd+bwd, {N} Gaussians, {shape[0]}x{shape[1]}, tile "
                                f"{args.tile}", "num_gauss": N, "width": shape[0],
-                   "height": shape[1], "tile": args.tile, "threads": args.threads,
+                   "height": shape[1], "tile": args.tile, "threads_fwd": args.threads,
+                   "threads_bwd": args.threads_bwd,
                    "pairs_K": K, "parallelism": "single GPU",
                    "autotune_ms": {k: round(v, 3) for k, v in
                                    (getattr(args, "autotune", None) or {}).items()}},

@@ -59,6 +59,38 @@ struct ArrayLoader {
 
 __device__ inline float fast_exp(float x) { return __expf(x); }
 
+// Pixel -> thread mapping of a TS x TS tile worked by NT threads with PPT =
+// TS^2/NT pixels each: every wave owns a compact RW x RH region (8x8, 16x8
+// or 16x16) and every lane a QX x QY pixel quad in it, so box tests are
+// coherent within a wave and a whole wave can skip an entry whose box misses
+// its region.
+template <int TS, int NT>
+struct PixMap {
+  static constexpr int PPT = TS * TS / NT;
+  static constexpr int QX = PPT >= 2 ? 2 : 1, QY = PPT >= 4 ? 2 : 1;
+  static constexpr int RW = 8 * QX, RH = 8 * QY;
+  static constexpr int RX = TS / RW;
+  static_assert(PPT == QX * QY && TS % RW == 0 && (NT / 64) * RW * RH == TS * TS,
+                "unsupported tile / workgroup shape");
+  // wave-uniform region origin (in tile-local pixels)
+  __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+  __device__ static uint32_t rx0() { return (wave() % RX) * RW; }
+  __device__ static uint32_t ry0() { return (wave() / RX) * RH; }
+  __device__ static uint32_t lx(int k) {
+    return rx0() + (threadIdx.x & 7) * QX + uint32_t(k % QX);
+  }
+  __device__ static uint32_t ly(int k) {
+    return ry0() + ((threadIdx.x & 63) >> 3) * QY + uint32_t(k / QX);
+  }
+};
+
+// Whether the box (x0, y0, x1, y1) can contain a pixel centre of the
+// wave's region [cx0, cx0 + RW - 1] x [cy0, cy0 + RH - 1] (centres).
+__device__ inline bool region_hit(float bx0, float by0, float bx1, float by1, float cx0,
+                                  float cy0, float cx1, float cy1) {
+  return !(bx1 < cx0 || bx0 > cx1 || by1 < cy0 || by0 > cy1);
+}
+
 // Forward of one tile. Image arrays are the view's base pointers (pixel
 // index iy * img_w + ix); alpha / nwalk / tlast may be null. The entry loop is
 // outermost so one LDS read of a record serves all PPT pixels of a thread.
@@ -71,17 +103,20 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                                                 float* __restrict__ pix2alpha,
                                                 uint32_t* __restrict__ pix2nwalk,
                                                 float* __restrict__ pix2tlast) {
-  constexpr int PPT = TS * TS / NT;
+  using PM = PixMap<TS, NT>;
+  constexpr int PPT = PM::PPT;
   __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
   const uint32_t num = end - start;
   float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
   uint32_t walked[PPT];
   bool done[PPT];
   int live = 0;
+  // pixel-centre bounds of the wave's region
+  const float cx0 = float(tx * TS + PM::rx0()) + 0.5f, cy0 = float(ty * TS + PM::ry0()) + 0.5f;
+  const float cx1 = cx0 + float(PM::RW - 1), cy1 = cy0 + float(PM::RH - 1);
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t lp = threadIdx.x + k * NT;
-    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
     cr[k] = cg[k] = cbl[k] = asum[k] = 0.f;
@@ -102,46 +137,42 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     }
     __syncthreads();
     const uint32_t cnt = min(uint32_t(NT), num - b);
-    for (uint32_t j = 0; j < cnt && live > 0; ++j) {
+    // Branch-free per pixel: a splat that misses the pixel (or a finished
+    // pixel) gets e = 0, which leaves every accumulator bit-identical
+    // (x + 0 = x, T * 1 = T); only whole-wave skips branch.
+    for (uint32_t j = 0; j < cnt; ++j) {
+      if (__ballot(live > 0) == 0) break;  // every pixel of the wave is done
       const float4 q2 = s2[j];
       const float4 q3 = s3[j];
-      bool in[PPT];
-      bool anyin = false;
+      const float4 q0 = s0[j];
+      const float4 q1 = s1[j];
+      if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
 #pragma unroll
       for (int k = 0; k < PPT; ++k) {
         // inclusive box test (aabb2::is_include_point)
-        in[k] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
-        anyin = anyin || in[k];
-      }
-      if (!anyin) continue;
-      const float4 q0 = s0[j];
-      const float4 q1 = s1[j];
-#pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        if (!in[k]) continue;
+        const bool in =
+            !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
         const float dx = px[k] - q0.x, dy = py[k] - q0.y;
         const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
-        const float e = fast_exp(-0.5f * q) * q1.y;
+        const float e = in ? fast_exp(-0.5f * q) * q1.y : 0.f;
         const float eo = tr[k] * e;
         cr[k] += q1.z * eo;
         cg[k] += q1.w * eo;
         cbl[k] += q2.x * eo;
-        tl[k] = tr[k];
+        tl[k] = in ? tr[k] : tl[k];
         tr[k] *= 1.f - e;
         asum[k] += eo;
-        if (asum[k] >= 0.999f) {
-          done[k] = true;
-          walked[k] = b + j + 1;
-          --live;
-        }
+        const bool term = in && asum[k] >= 0.999f;
+        walked[k] = term ? b + j + 1 : walked[k];
+        done[k] = done[k] || term;
+        live -= term ? 1 : 0;
       }
     }
     if (__syncthreads_and(live == 0)) break;
   }
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t lp = threadIdx.x + k * NT;
-    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
     if (ix >= img_w || iy >= img_h) continue;
     const size_t ip = size_t(iy) * img_w + ix;
     pix2rgb[3 * ip + 0] = cr[k];
@@ -222,7 +253,10 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out,
                                                 BwdLds<NT>& L) {
-  constexpr int PPT = TS * TS / NT;
+  using PM = PixMap<TS, NT>;
+  constexpr int PPT = PM::PPT;
+  const float cx0 = float(tx * TS + PM::rx0()) + 0.5f, cy0 = float(ty * TS + PM::ry0()) + 0.5f;
+  const float cx1 = cx0 + float(PM::RW - 1), cy1 = cy0 + float(PM::RH - 1);
   float4* s0 = L.s0;
   float4* s1 = L.s1;
   float4* s2 = L.s2;
@@ -238,8 +272,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   uint32_t my_max = 0;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t lp = threadIdx.x + k * NT;
-    const uint32_t ix = tx * TS + lp % TS, iy = ty * TS + lp / TS;
+    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
     first[k] = true;
@@ -288,25 +321,28 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     __syncthreads();
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t it = hi - 1 - j;
-      const float4 q0 = s0[j], q1 = s1[j], q2 = s2[j], q3 = s3[j];
+      const float4 q2 = s2[j], q3 = s3[j];
+      const float4 q0 = s0[j], q1 = s1[j];
+      if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
       bool any = false;
+      // Branch-free per pixel: a splat that misses the pixel gets gk = e = 0,
+      // which adds exact zeros and keeps T (T * rcp(1) = T).
 #pragma unroll
       for (int k = 0; k < PPT; ++k) {
         const bool in = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
                         py[k] <= q3.x;
-        if (!in) continue;
-        any = true;
+        any = any || in;
         const float dx = px[k] - q0.x, dy = py[k] - q0.y;
         const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
-        const float gk = fast_exp(-0.5f * q);
+        const float gk = in ? fast_exp(-0.5f * q) : 0.f;
         const float e = gk * q1.y;
         const float ome = 1.f - e;
         const float inv = __builtin_amdgcn_rcpf(ome);
-        if (!first[k]) tr[k] = tr[k] * inv;
-        first[k] = false;
+        tr[k] = first[k] ? tr[k] : tr[k] * inv;
+        first[k] = first[k] && !in;
         const float t = tr[k];
         float dl_de = g0[k] * (q1.z * t - b0[k] * inv) + g1[k] * (q1.w * t - b1[k] * inv) +
                       g2[k] * (q2.x * t - b2[k] * inv);

@@ -151,7 +151,8 @@ def bin_gauss(img_shape, tile_size, pxd, aabb):
 
 class _Render(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, xyz, log_scale, quat, opacity, rgb, m16, img_shape, tile_size, threads):
+    def forward(ctx, xyz, log_scale, quat, opacity, rgb, m16, img_shape, tile_size, threads,
+                threads_bwd):
         xyz, log_scale, quat, opacity, rgb = (t.contiguous() for t in
                                               (xyz, log_scale, quat, opacity, rgb))
         pxd, conic, aabb, splat2 = project_fwd(xyz, log_scale, quat, opacity, rgb, m16,
@@ -162,7 +163,7 @@ class _Render(torch.autograd.Function):
         ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast)
         ctx.img_shape = img_shape
         ctx.tile_size = tile_size
-        ctx.threads = threads
+        ctx.threads = threads_bwd
         ctx.mark_non_differentiable(nwalk)
         return img, alpha
 
@@ -176,17 +177,19 @@ class _Render(torch.autograd.Function):
                                                       d_alpha, ctx.threads)
         d_xyz, d_ls, d_quat = project_bwd(xyz, log_scale, quat, m16, ctx.img_shape, d_pixxy,
                                           d_conic, log_scale=True)
-        return d_xyz, d_ls, d_quat, d_op, d_rgb, None, None, None, None
+        return d_xyz, d_ls, d_quat, d_op, d_rgb, None, None, None, None, None
 
 
 def render(xyz, log_scale, quat, opacity, rgb, transform_world2ndc, img_shape,
-           tile_size=DEFAULT_TILE, threads=0):
+           tile_size=DEFAULT_TILE, threads=0, threads_bwd=None):
     """Differentiable render of N Gaussians -> (rgb [H,W,3], alpha [H,W]).
 
     xyz [N,3] means, log_scale [N,3] log sigma, quat [N,4] (x,y,z,w; used as
     given), opacity [N] in (0,1), rgb [N,3]; transform_world2ndc (4,4)
     row-major or (16,) column-major; img_shape (W, H). Gradients flow to all
-    five per-Gaussian inputs through the adjoint kernels.
+    five per-Gaussian inputs through the adjoint kernels. threads /
+    threads_bwd: workgroup size of the forward / adjoint raster (0: default;
+    threads_bwd defaults to threads).
     """
     require_gpu(xyz, "xyz")
     n = xyz.shape[0]
@@ -199,4 +202,4 @@ def render(xyz, log_scale, quat, opacity, rgb, transform_world2ndc, img_shape,
         raise ValueError("tile_size must be 8, 16 or 32")
     m16 = world2ndc_col_major(transform_world2ndc, xyz.device)
     return _Render.apply(xyz, log_scale, quat, opacity, rgb, m16, tuple(img_shape), tile_size,
-                         threads)
+                         threads, threads if threads_bwd is None else threads_bwd)

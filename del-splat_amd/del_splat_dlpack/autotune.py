@@ -37,39 +37,60 @@ def _store(db):
 
 def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=None,
                 use_cache=True):
-    """Single-view forward + L2 adjoint (the differentiable render path).
-    Returns ((tile_size, threads), {config: ms})."""
+    """Single-view forward + L2 adjoint (the differentiable render path). Per
+    tile size the forward and the adjoint workgroup sizes are timed
+    separately (the kernels are independent; the forward result does not
+    depend on its workgroup size). Returns ((tile_size, threads_fwd,
+    threads_bwd), {config: ms})."""
     from .SplatGauss import torch as SG
 
-    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd")
+    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd_v2")
     db = _load() if use_cache else {}
     if sig in db:
         e = db[sig]
-        return (e["best"][0], e["best"][1]), dict(e["ms"])
+        return tuple(e["best"]), dict(e["ms"])
     xyz, ls, q, op, rgb = tensors
-    res = {}
-    for ts, th in candidates or CANDIDATES_BWD:
-        def step():
-            pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, img_shape)
-            t2i, i2p = SG.bin_gauss(img_shape, ts, pxd, aabb)
-            img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th)
-            d = SG.rasterize_bwd(img_shape, ts, splat2, t2i, i2p, nwalk, tlast,
-                                 2.0 * (img - target), None, th)
-            SG.project_bwd(xyz, ls, q, m16, img_shape, d[0], d[1])
+    fwd_opts = {8: (64,), 16: (64, 128, 256), 32: (256, 512, 1024)}
+    bwd_opts = {8: (64,), 16: (64, 128, 256), 32: (256,)}
 
-        step()
+    def timed(fn):
+        fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            step()
+            fn()
         torch.cuda.synchronize()
-        res[f"{ts}/{th}"] = (time.perf_counter() - t0) * 1000.0 / reps
-    best = min(res, key=res.get)
-    ts, th = (int(x) for x in best.split("/"))
-    db[sig] = {"best": [ts, th], "ms": res}
+        return (time.perf_counter() - t0) * 1000.0 / reps
+
+    res, best = {}, None
+    for ts in (candidates or (8, 16, 32)):
+        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, img_shape)
+        t_front = timed(lambda: SG.bin_gauss(img_shape, ts, *SG.project_fwd(
+            xyz, ls, q, op, rgb, m16, img_shape)[::2]))
+        t2i, i2p = SG.bin_gauss(img_shape, ts, pxd, aabb)
+        tf = {th: timed(lambda th=th: SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th))
+              for th in fwd_opts[ts]}
+        thf = min(tf, key=tf.get)
+        img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, thf)
+        d_img = 2.0 * (img - target)
+        tb = {th: timed(lambda th=th: SG.rasterize_bwd(img_shape, ts, splat2, t2i, i2p, nwalk,
+                                                      tlast, d_img, None, th))
+              for th in bwd_opts[ts]}
+        thb = min(tb, key=tb.get)
+        total = t_front + tf[thf] + tb[thb]
+        for th, v in tf.items():
+            res[f"{ts}/fwd{th}"] = v
+        for th, v in tb.items():
+            res[f"{ts}/bwd{th}"] = v
+        res[f"{ts}/total"] = total
+        if best is None or total < best[0]:
+            best = (total, ts, thf, thb)
+        del pxd, conic, aabb, splat2, t2i, i2p, img, alpha, nwalk, tlast, d_img
+    _, ts, thf, thb = best
+    db[sig] = {"best": [ts, thf, thb], "ms": res}
     if use_cache:
         _store(db)
-    return (ts, th), res
+    return (ts, thf, thb), res
 
 
 def tune_multiview(num_pnt, views, tensors, mode="forward", target=None, reps=3,
