@@ -38,11 +38,16 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--tile", type=int, default=0, help="0: autotune tile size and threads")
-    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0, help="forward raster workgroup size")
+    ap.add_argument("--threads-bwd", type=int, default=None,
+                    help="adjoint raster workgroup size (default: --threads)")
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--train-gauss", type=int, default=3_000_000)
+    ap.add_argument("--train-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB megabytes (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -94,7 +99,8 @@ def headline(args, torch, np, SG, scene):
     m16 = t(m)
     target = t(scene.target_image(shape, 102))
     tune = None
-    args.threads_bwd = args.threads
+    if args.threads_bwd is None:
+        args.threads_bwd = args.threads
     if ts == 0:
         from del_splat_dlpack import autotune
         (ts, args.threads, args.threads_bwd), tune = autotune.tune_single(
@@ -222,6 +228,90 @@ def batched(args, torch, np, SG, scene):
     return out
 
 
+def train(args, torch, np, SG, scene):
+    """Training workload: 3M Gaussians, 8 uneven views per step drawn afresh
+    each step (cameras as in the batched section, per-step seed), fused
+    forward + L2 adjoint over all views + Adam on all 14N parameters. Reports
+    ms/step and the gradient agreement of the batched path with the sum of
+    per-view single-view gradients on the first step."""
+    from del_splat_dlpack import _native
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
+    dev = torch.device("cuda:0")
+    n, nview, steps = args.train_gauss, 8, args.train_steps
+    s = scene.gaussians(n, 4, radius=1.0)
+    logit = np.log(s["opacity"] / (1.0 - s["opacity"])).astype(np.float32)
+    flat = np.concatenate([s["xyz"].reshape(-1), s["log_scale"].reshape(-1),
+                           s["quat"].reshape(-1), logit, s["rgb"].reshape(-1)])
+    params = torch.from_numpy(flat).to(dev)
+    grads = torch.zeros_like(params)
+    m1 = torch.zeros_like(params)
+    m2 = torch.zeros_like(params)
+    dstep = torch.zeros(1, device=dev, dtype=torch.int32)
+    max_pix = nview * 1920 * 1080
+    plan = None
+    ts, th = 32, 256
+
+    def views_for(step):
+        return scene.sphere_cameras(nview, 1000 + step)
+
+    # gradient agreement on step 0: batched adjoint vs the sum over views of
+    # the single-view adjoint (both before any update)
+    views = views_for(0)
+    plan = MultiViewPlan(n, views, ts, th, max_pixels=max_pix,
+                         pair_capacity=int(n * nview * 14))
+    targets = torch.cat([torch.from_numpy(scene.target_image(sh, 5000 + v)).to(dev).reshape(-1)
+                         for v, (_, sh) in enumerate(views)])
+    xyz, ls, q = params[:3 * n].view(n, 3), params[3 * n:6 * n].view(n, 3), params[6 * n:10 * n].view(n, 4)
+    op = torch.sigmoid(params[10 * n:11 * n])
+    rgb = params[11 * n:].view(n, 3)
+    out, _ = plan.forward(xyz, ls, q, op, rgb)
+    k0, over = plan.counters()
+    if over:
+        raise RuntimeError("train plan capacity too small")
+    gb = plan.backward(xyz, ls, q, op, out_rgb=out, target=targets)
+    leaf = [t.clone().requires_grad_(True) for t in (xyz, ls, q, op, rgb)]
+    tsplit = plan.split(targets, 3)
+    total = 0.0
+    for v, (m, sh) in enumerate(views):
+        img, _ = SG.render(*leaf, torch.from_numpy(m).to(dev), sh, tile_size=ts, threads=th)
+        total = total + ((img - tsplit[v]) ** 2).sum()
+    total.backward()
+    agree = {}
+    for name, a, b in (("xyz", gb["xyz"], leaf[0].grad), ("log_scale", gb["scale"], leaf[1].grad),
+                       ("quat", gb["quat"], leaf[2].grad), ("opacity", gb["opacity"], leaf[3].grad),
+                       ("rgb", gb["rgb"], leaf[4].grad)):
+        agree[name] = float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
+    del leaf, total, gb
+    # timed training loop (eager steps: the view set changes every step)
+    losses = []
+    out = torch.empty(max_pix * 3, device=dev)
+    gen = torch.Generator(device=dev)
+    targets = torch.empty(max_pix * 3, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(steps):
+        views = views_for(step)
+        plan.set_views(views)
+        # seeded U[0,1] target images, generated on the GPU
+        gen.manual_seed(5000 + step)
+        torch.rand(max_pix * 3, device=dev, generator=gen, out=targets)
+        plan.train_step(params, grads, m1, m2, dstep, targets, out, lr=1e-3)
+        if step in (0, steps - 1):
+            n3 = plan.total_pix * 3
+            losses.append(float(((out[:n3] - targets[:n3]) ** 2).mean()))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    k, over = plan.counters()
+    plan.close()
+    torch.cuda.empty_cache()
+    return {"num_gauss": n, "views_per_step": nview, "steps": steps, "ms_per_step": round(ms, 3),
+            "tile": ts, "threads": th, "pairs_K_step0": k0, "overflow": over,
+            "grad_max_rel_diff_vs_per_view": {k2: round(v, 7) for k2, v in agree.items()},
+            "loss_first_last": losses,
+            "note": "step time includes the per-step host work (view table upload, target "
+                    "image generation and upload)"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -251,8 +341,11 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs,
+        # keyed by kernel and tile size (the traffic depends on the schedule)
         try:
-            traffic = json.load(open(tpath)).get(dom)
+            entry = json.load(open(tpath)).get(f"{dom}@tile{args.tile}")
+            traffic = None if entry is None else entry["bytes"]
         except Exception:
             traffic = None
     out = {
@@ -282,6 +375,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(args, torch, np, SG, scene)
+    if not args.no_train:
+        out["train"] = train(args, torch, np, SG, scene)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s, m, shape, args.tile)
     print(json.dumps(out), flush=True)

@@ -37,16 +37,13 @@ template <typename Boxes>
 __global__ void __launch_bounds__(256)
 bin_count_kernel(uint32_t n, Boxes boxes, const float* __restrict__ pixxydepth, uint32_t ts,
                  uint32_t tw, uint32_t th, uint32_t* __restrict__ cnt,
-                 uint32_t* __restrict__ dkey, uint32_t* __restrict__ order) {
+                 uint32_t* __restrict__ dkey) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 b = boxes(i);
   const TileRect r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
   cnt[i] = uint32_t(r.count());
-  if (dkey) {
-    dkey[i] = depth_key(pixxydepth[3 * i + 2]);
-    order[i] = i;
-  }
+  if (dkey) dkey[i] = depth_key(pixxydepth[3 * i + 2]);
 }
 
 // cnt_sorted[j] = cnt[order[j]]
@@ -161,7 +158,7 @@ int count_pairs(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw
     return DSPLAT_OK;
   }
   bin_count_kernel<Boxes><<<div_up(n, 256), 256, 0, s>>>(n, boxes, nullptr, ts, tw, th, cnt,
-                                                         nullptr, nullptr);
+                                                         nullptr);
   // reduce through the scan (out == in is allowed; cnt is scratch here)
   int rc = exclusive_scan(s, n, nullptr, cnt, cnt, scan_ws, d_num_idx);
   if (rc) return rc;
@@ -182,12 +179,12 @@ int bin_and_sort(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth
     set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
   } else {
     const uint32_t nb = div_up(n, 256);
-    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt, L.dkey,
-                                               L.order);
+    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt, L.dkey);
     bool in_alt = false;
     uint32_t *kalt = nullptr, *valt = nullptr;
+    // values: Gaussian indices (iota), generated by the first pass
     int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
-                                        L.dsort_bytes, &in_alt, &kalt, &valt);
+                                        L.dsort_bytes, &in_alt, &kalt, &valt, true);
     if (rc) return rc;
     const uint32_t* order = in_alt ? valt : L.order;
     gather_u32_kernel<<<nb, 256, 0, s>>>(n, order, L.cnt, L.off);

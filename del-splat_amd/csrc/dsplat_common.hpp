@@ -21,7 +21,9 @@ namespace dsplat {
 
 constexpr int kWave = 64;
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
+__host__ __device__ inline uint32_t div_up(uint64_t a, uint64_t b) {
+  return uint32_t((a + b - 1) / b);
+}
 
 // Tile grid of an image (ceil, as splat_gauss.rs:59-62).
 inline void tile_grid(uint32_t w, uint32_t h, uint32_t ts, uint32_t* tw, uint32_t* th) {

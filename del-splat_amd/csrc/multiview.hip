@@ -56,8 +56,7 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
                   const float* __restrict__ xyz, const float* __restrict__ scale,
                   const float4* __restrict__ quat, const float* __restrict__ opacity,
                   const float* __restrict__ rgb, float4* __restrict__ rec,
-                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ dkey,
-                  uint32_t* __restrict__ order) {
+                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ dkey) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t v = blockIdx.y;
@@ -83,7 +82,6 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
   const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
   cnt[item] = uint32_t(tr.count());
   dkey[item] = depth_key(p.ndcz);
-  order[item] = uint32_t(item);
 }
 
 // ------------------------------------------------------------- binning ---
@@ -511,7 +509,7 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   const float4* q4 = reinterpret_cast<const float4*>(quat);
 #define DSPLAT_MVP(A, B)                                                                    \
   mv_project_kernel<A, B><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
-                                             rgb, P->rec, P->cnt, P->dkey, P->order)
+                                             rgb, P->rec, P->cnt, P->dkey)
   if (log_scale && logit) DSPLAT_MVP(true, true);
   else if (log_scale) DSPLAT_MVP(true, false);
   else if (logit) DSPLAT_MVP(false, true);
@@ -519,8 +517,9 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
 #undef DSPLAT_MVP
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
+  // values: list items (iota), generated by the first pass
   int rc = radix_sort_pairs<uint32_t>(s, items, nullptr, P->dkey, P->order, 31, P->dsort_ws,
-                                      P->dsort_bytes, &in_alt, &kalt, &valt);
+                                      P->dsort_bytes, &in_alt, &kalt, &valt, true);
   if (rc) return rc;
   const uint32_t* order = in_alt ? valt : P->order;
   mv_gather_kernel<<<div_up(items, 256), 256, 0, s>>>(items, order, P->cnt, P->off);

@@ -3,8 +3,8 @@
 // sort_by_key_u64 and array1d::fill used by the reference host layer
 // (del-splat-cudarc/src/splat_gauss.rs:137-146,184-191).
 //
-// Radix sort: 8-bit digits, one (upsweep, scan, scatter) triple per pass.
-// A 256-thread workgroup owns a 2048-element tile; each wave64 owns 512
+// Radix sort: 8-bit digits, one (upsweep, column scan, scatter) triple per
+// pass. A 256-thread workgroup owns a 2048-element tile; each wave64 owns 512
 // consecutive elements and ranks them in rounds of 64 with a ballot-based
 // digit match, keeping per-wave digit counters in LDS, so the rank of an
 // element is (digit offset of earlier tiles) + (count in earlier waves of the
@@ -136,48 +136,125 @@ int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uin
 }
 
 // ------------------------------------------------------------ radix sort ----
+// Per pass three kernels: upsweep (per-tile digit counts), column scan (one
+// workgroup per digit: the prefix of the digit's counts over the tiles, and
+// the digit's total), scatter (digit bases from the totals, then the stable
+// in-tile rank).
 constexpr int kSortBlock = 256;
 constexpr int kSortItems = 8;                        // rounds per wave
 constexpr int kSortWaveChunk = 64 * kSortItems;      // 512
 constexpr int kSortTile = kSortBlock * kSortItems;   // 2048
 constexpr int kSortWaves = kSortBlock / 64;
 
+struct SortLayout {
+  void* kalt;
+  uint32_t* valt;
+  uint32_t* tot;   // [256] digit totals of the current pass
+  uint32_t* cnt;   // [256][tiles] per-tile digit counts, then their prefixes
+  size_t total;
+};
+
 template <typename KeyT>
-__global__ void __launch_bounds__(kSortBlock)
-radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ keys,
-                     uint32_t shift, uint32_t nbins, uint32_t nblocks,
-                     uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
-  const uint32_t n = load_n(n_cap, d_n);
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
-  const uint32_t mask = nbins - 1;
-  for (int k = 0; k < kSortItems; ++k) {
-    const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & mask], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x < nbins) hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+SortLayout sort_layout(void* base, uint32_t n) {
+  const uint32_t nb = div_up(n ? n : 1, kSortTile);
+  SortLayout L{};
+  char* p = static_cast<char*>(base);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    char* q = p ? p + o : nullptr;
+    o += (bytes + 255) & ~size_t(255);
+    return q;
+  };
+  L.kalt = take(size_t(n) * sizeof(KeyT));
+  L.valt = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
+  L.tot = reinterpret_cast<uint32_t*>(take(size_t(256) * 4));
+  L.cnt = reinterpret_cast<uint32_t*>(take(size_t(256) * nb * 4));
+  L.total = o;
+  return L;
 }
 
 template <typename KeyT>
 __global__ void __launch_bounds__(kSortBlock)
+radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ keys,
+                     uint32_t shift, uint32_t dbits, uint32_t nb, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[256];
+  const uint32_t n = load_n(n_cap, d_n);
+  const int lane = threadIdx.x & 63;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+  const uint32_t mask = (1u << dbits) - 1u;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t d = valid ? uint32_t(keys[i] >> shift) & mask : 0u;
+    const uint64_t act = __ballot(valid);
+    if (act == 0) continue;  // wave-uniform
+    // valid lanes are a prefix of the wave, so lane 0 is one of them; a
+    // digit shared by the whole wave takes one LDS add, not 64 conflicting
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__ballot(valid && d != d0) == 0) {
+      if (lane == 0) atomicAdd(&h[d0], uint32_t(__popcll(act)));
+    } else if (valid) {
+      atomicAdd(&h[d], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x <= mask) cnt[size_t(threadIdx.x) * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Workgroup d: cnt[d][b] <- elements of digit d in tiles before b;
+// tot[d] <- elements of digit d.
+__global__ void __launch_bounds__(kSortBlock)
+radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
+                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ tot) {
+  const uint32_t n = load_n(n_cap, d_n);
+  const uint32_t nbv = div_up(n, kSortTile);
+  const uint32_t d = blockIdx.x;
+  uint32_t carry = 0;
+  uint32_t* col = cnt + size_t(d) * nb;
+  for (uint32_t b0 = 0; b0 < nbv; b0 += kSortTile) {
+    const uint32_t i0 = b0 + threadIdx.x * kSortItems;
+    uint32_t v[kSortItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+      v[k] = i0 + k < nbv ? col[i0 + k] : 0u;
+      s += v[k];
+    }
+    uint32_t chunk;
+    uint32_t run = carry + block_exclusive_scan(s, &chunk);
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+      if (i0 + k < nbv) col[i0 + k] = run;
+      run += v[k];
+    }
+    carry += chunk;
+  }
+  if (threadIdx.x == 0) tot[d] = carry;
+}
+
+// iota: the values are the element indices (vin is not read).
+template <typename KeyT>
+__global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
-                     const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
+                     const uint32_t* __restrict__ vin, bool iota, KeyT* __restrict__ kout,
                      uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
-                     uint32_t nblocks, const uint32_t* __restrict__ hist) {
+                     uint32_t nb, const uint32_t* __restrict__ off,
+                     const uint32_t* __restrict__ tot, uint32_t nbins) {
   static_assert(kSortBlock == 256, "one thread per digit");
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dpre[256];
+  __shared__ uint32_t dbase[256];
   __shared__ KeyT skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   const uint32_t n = load_n(n_cap, d_n);
+  const uint64_t bstart = uint64_t(blockIdx.x) * kSortTile;
+  if (bstart >= n) return;  // uniform
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile + uint64_t(w) * kSortWaveChunk;
+  const uint64_t base = bstart + uint64_t(w) * kSortWaveChunk;
   KeyT key[kSortItems];
   uint32_t val[kSortItems], rank[kSortItems], dig[kSortItems];
 #pragma unroll
@@ -185,7 +262,7 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     const uint64_t i = base + uint64_t(r) * 64 + lane;
     const bool valid = i < n;
     key[r] = valid ? kin[i] : KeyT(0);
-    val[r] = valid ? vin[i] : 0u;
+    val[r] = valid ? (iota ? uint32_t(i) : vin[i]) : 0u;
     const uint32_t d = uint32_t(key[r] >> shift) & mask;
     dig[r] = valid ? d : 0xffffffffu;
     uint64_t peers = __ballot(valid);
@@ -202,21 +279,24 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   }
   __syncthreads();
   // exclusive prefix of the per-wave counts over waves, per digit, and the
-  // block's digit totals
+  // tile's digit totals
   uint32_t total = 0;
-  {
 #pragma unroll
-    for (int k = 0; k < kSortWaves; ++k) {
-      const uint32_t c = wcnt[k][threadIdx.x];
-      wcnt[k][threadIdx.x] = total;
-      total += c;
-    }
+  for (int k = 0; k < kSortWaves; ++k) {
+    const uint32_t c = wcnt[k][threadIdx.x];
+    wcnt[k][threadIdx.x] = total;
+    total += c;
   }
-  // dpre[d]: elements of the block with a smaller digit
-  uint32_t btot;
+  // dpre[d]: elements of the tile with a smaller digit; dbase[d]: elements
+  // of the whole array with a smaller digit, plus those of digit d in
+  // earlier tiles
+  uint32_t btot, all;
   dpre[threadIdx.x] = block_exclusive_scan(total, &btot);
+  const bool live_digit = threadIdx.x < nbins;
+  const uint32_t gpre = block_exclusive_scan(live_digit ? tot[threadIdx.x] : 0u, &all);
+  if (live_digit) dbase[threadIdx.x] = gpre + off[size_t(threadIdx.x) * nb + blockIdx.x];
   __syncthreads();
-  // stage in LDS in block-sorted order ...
+  // stage in LDS in tile-sorted order ...
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     if (dig[r] == 0xffffffffu) continue;
@@ -227,12 +307,11 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   }
   __syncthreads();
   // ... and write each digit's run to consecutive global addresses
-  const uint64_t bstart = uint64_t(blockIdx.x) * kSortTile;
-  const uint32_t nb_valid = bstart >= n ? 0u : uint32_t(min(uint64_t(kSortTile), n - bstart));
-  for (uint32_t i = threadIdx.x; i < nb_valid; i += kSortBlock) {
+  const uint32_t nvalid = uint32_t(min(uint64_t(kSortTile), n - bstart));
+  for (uint32_t i = threadIdx.x; i < nvalid; i += kSortBlock) {
     const KeyT k = skey[i];
     const uint32_t d = uint32_t(k >> shift) & mask;
-    const uint32_t pos = hist[d * nblocks + blockIdx.x] + (i - dpre[d]);
+    const uint32_t pos = dbase[d] + (i - dpre[d]);
     kout[pos] = k;
     vout[pos] = sval[i];
   }
@@ -240,46 +319,37 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
 
 template <typename KeyT>
 size_t sort_workspace_bytes(uint32_t n) {
-  const uint32_t nb = div_up(n ? n : 1, kSortTile);
-  size_t b = 0;
-  b += (size_t(n) * sizeof(KeyT) + 255) & ~size_t(255);         // alternate keys
-  b += (size_t(n) * sizeof(uint32_t) + 255) & ~size_t(255);     // alternate values
-  b += (size_t(256) * nb * sizeof(uint32_t) + 255) & ~size_t(255);  // histograms
-  b += scan_workspace_bytes(256u * nb);
-  return b;
+  return sort_layout<KeyT>(nullptr, n).total;
 }
 
 template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
-                     bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out) {
+                     bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out,
+                     bool iota_vals) {
   *result_in_alt = false;
   if (ws_bytes < sort_workspace_bytes<KeyT>(n_cap))
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
   if (key_bits == 0 || key_bits > 8 * sizeof(KeyT))
     return dsplat_set_error(DSPLAT_ERR_ARG, "radix sort: key_bits out of range");
-  const uint32_t nb = div_up(n_cap ? n_cap : 1, kSortTile);
-  char* p = static_cast<char*>(ws);
-  KeyT* kalt = reinterpret_cast<KeyT*>(p);
-  p += (size_t(n_cap) * sizeof(KeyT) + 255) & ~size_t(255);
-  uint32_t* valt = reinterpret_cast<uint32_t*>(p);
-  p += (size_t(n_cap) * sizeof(uint32_t) + 255) & ~size_t(255);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(p);
-  p += (size_t(256) * nb * sizeof(uint32_t) + 255) & ~size_t(255);
-  uint32_t* scan_ws = reinterpret_cast<uint32_t*>(p);
+  const SortLayout L = sort_layout<KeyT>(ws, n_cap);
+  KeyT* kalt = static_cast<KeyT*>(L.kalt);
   if (keys_alt_out) *keys_alt_out = kalt;
-  if (vals_alt_out) *vals_alt_out = valt;
+  if (vals_alt_out) *vals_alt_out = L.valt;
   if (n_cap == 0) return DSPLAT_OK;
-  KeyT* ksrc = keys; uint32_t* vsrc = vals; KeyT* kdst = kalt; uint32_t* vdst = valt;
+  const uint32_t nb = div_up(n_cap, kSortTile);
+  const uint32_t P = div_up(key_bits, 8u);
+  KeyT* ksrc = keys; uint32_t* vsrc = vals; KeyT* kdst = kalt; uint32_t* vdst = L.valt;
   bool in_alt = false;
-  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-    const uint32_t dbits = key_bits - shift < 8 ? key_bits - shift : 8;
-    const uint32_t nbins = 1u << dbits;
-    radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, nbins, nb, hist);
-    int rc = exclusive_scan(s, nbins * nb, nullptr, hist, hist, scan_ws, nullptr);
-    if (rc) return rc;
-    radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, vsrc, kdst, vdst,
-                                                         shift, dbits, nb, hist);
+  for (uint32_t p = 0; p < P; ++p) {
+    const uint32_t shift = 8u * p;
+    const uint32_t dbits = min(8u, key_bits - shift);
+    radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb,
+                                                         L.cnt);
+    radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, L.cnt, L.tot);
+    radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(
+        n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt, L.tot,
+        1u << dbits);
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;
@@ -293,10 +363,10 @@ template size_t sort_workspace_bytes<uint32_t>(uint32_t);
 template size_t sort_workspace_bytes<uint64_t>(uint32_t);
 template int radix_sort_pairs<uint32_t>(hipStream_t, uint32_t, const uint32_t*, uint32_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint32_t**,
-                                        uint32_t**);
+                                        uint32_t**, bool);
 template int radix_sort_pairs<uint64_t>(hipStream_t, uint32_t, const uint32_t*, uint64_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint64_t**,
-                                        uint32_t**);
+                                        uint32_t**, bool);
 
 // ------------------------------------------------------------------ fill ----
 __global__ void fill_f32_kernel(uint32_t n, float* __restrict__ buf, float v) {

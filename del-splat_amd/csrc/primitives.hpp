@@ -19,12 +19,16 @@ template <typename KeyT>
 size_t sort_workspace_bytes(uint32_t n);
 
 // Stable LSD radix sort of (keys, vals) over the low key_bits bits, n =
-// min(*d_n, n_cap). The result is in (keys, vals) if *result_in_alt is false,
-// else in the workspace's alternate buffers (*keys_alt, *vals_alt).
+// min(*d_n, n_cap). The result is in (keys, vals) if
+// *result_in_alt is false, else in the workspace's alternate buffers
+// (*keys_alt, *vals_alt). iota_vals: the values are the indices 0..n-1 and
+// vals is not read (it still receives the result when *result_in_alt is
+// false).
 template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
-                     bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt);
+                     bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt,
+                     bool iota_vals = false);
 
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 

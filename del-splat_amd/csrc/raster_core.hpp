@@ -147,11 +147,18 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       const float4 q0 = s0[j];
       const float4 q1 = s1[j];
       if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
+      bool inm[PPT];
+      bool any = false;
 #pragma unroll
       for (int k = 0; k < PPT; ++k) {
         // inclusive box test (aabb2::is_include_point)
-        const bool in =
-            !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
+        inm[k] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
+        any = any || inm[k];
+      }
+      if (__ballot(any) == 0) continue;  // no live pixel of the wave is inside the box
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        const bool in = inm[k];
         const float dx = px[k] - q0.x, dy = py[k] - q0.y;
         const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
         const float e = in ? fast_exp(-0.5f * q) * q1.y : 0.f;
@@ -266,7 +273,9 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   uint32_t& smax = L.smax;
   const int lane = threadIdx.x & 63;
   float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
-  float b0[PPT], b1[PPT], b2[PPT], ba[PPT];
+  // gb = g . (colour behind) + ga * (alpha behind): the back accumulators
+  // only enter dL/de through this dot product
+  float gb[PPT];
   uint32_t nw[PPT];
   bool first[PPT];
   uint32_t my_max = 0;
@@ -276,7 +285,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
     first[k] = true;
-    b0[k] = b1[k] = b2[k] = ba[k] = 0.f;
+    gb[k] = 0.f;
     if (ix < img_w && iy < img_h) {
       const size_t ip = size_t(iy) * img_w + ix;
       nw[k] = pix2nwalk[ip];
@@ -298,6 +307,11 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     }
     my_max = max(my_max, nw[k]);
   }
+  // walk bound of this wave (entries beyond it touch none of its pixels)
+  uint32_t wmax = my_max;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, uint32_t(__shfl_xor(int(wmax), off, 64)));
+  wmax = __builtin_amdgcn_readfirstlane(wmax);
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
   atomicMax(&smax, my_max);
@@ -321,20 +335,27 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     __syncthreads();
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t it = hi - 1 - j;
+      if (it >= wmax) continue;  // beyond every walk of the wave
       const float4 q2 = s2[j], q3 = s3[j];
       const float4 q0 = s0[j], q1 = s1[j];
       if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
+      bool inm[PPT];
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        inm[k] = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
+                 py[k] <= q3.x;
+        any = any || inm[k];
+      }
+      if (__ballot(any) == 0) continue;  // no pixel of the wave takes this entry
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
-      bool any = false;
       // Branch-free per pixel: a splat that misses the pixel gets gk = e = 0,
       // which adds exact zeros and keeps T (T * rcp(1) = T).
 #pragma unroll
       for (int k = 0; k < PPT; ++k) {
-        const bool in = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
-                        py[k] <= q3.x;
-        any = any || in;
+        const bool in = inm[k];
         const float dx = px[k] - q0.x, dy = py[k] - q0.y;
         const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
         const float gk = in ? fast_exp(-0.5f * q) : 0.f;
@@ -344,17 +365,15 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         tr[k] = first[k] ? tr[k] : tr[k] * inv;
         first[k] = first[k] && !in;
         const float t = tr[k];
-        float dl_de = g0[k] * (q1.z * t - b0[k] * inv) + g1[k] * (q1.w * t - b1[k] * inv) +
-                      g2[k] * (q2.x * t - b2[k] * inv);
-        if (kAlpha) dl_de += ga[k] * (t - ba[k] * inv);
+        // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)
+        float gc = g0[k] * q1.z + g1[k] * q1.w + g2[k] * q2.x;
+        if (kAlpha) gc += ga[k];
+        const float dl_de = t * gc - gb[k] * inv;
         const float et = e * t;
         v[6] += g0[k] * et;
         v[7] += g1[k] * et;
         v[8] += g2[k] * et;
-        b0[k] += q1.z * et;
-        b1[k] += q1.w * et;
-        b2[k] += q2.x * et;
-        if (kAlpha) ba[k] += et;
+        gb[k] += gc * et;
         v[5] += dl_de * gk;
         // moments of dL/dq: the mean and conic gradients of the entry are
         // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
@@ -366,7 +385,6 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         v[3] += a * dy;
         v[4] += bq * dy;
       }
-      if (__ballot(any) == 0) continue;  // wave-uniform
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) {
         const float r = wave_sum_to_last(v[c]);

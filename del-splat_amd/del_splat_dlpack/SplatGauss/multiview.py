@@ -25,6 +25,9 @@ def _view_array(views):
     return arr
 
 
+MAX_PAIRS = (1 << 32) - 1  # pair counts and capacities are 32-bit
+
+
 def pair_estimate(num_pnt, views, tile_size, mean_box_px=40.0):
     """A generous guess of the pair count K for the plan's capacity."""
     tiles_per = (mean_box_px / tile_size + 1.0) ** 2
@@ -41,7 +44,8 @@ class MultiViewPlan:
         self.max_view = max(len(views), 1)
         pix = sum(w * h for _, (w, h) in views)
         self.max_pixels = int(max_pixels or pix)
-        self.capacity = int(pair_capacity or pair_estimate(num_pnt, views, tile_size))
+        self.capacity = min(int(pair_capacity or pair_estimate(num_pnt, views, tile_size)),
+                            MAX_PAIRS)
         self._plan = ctypes.c_void_p()
         call("dsplat_mv_plan_create", self.num_pnt, self.max_view, self.max_pixels,
              self.tile_size, self.threads, self.capacity, ctypes.byref(self._plan))
@@ -183,14 +187,16 @@ class MultiViewPlan:
 
 def ensure_capacity(plan_factory, run, max_tries=3):
     """Runs `run(plan)` and grows the plan when the pair capacity overflowed.
-    plan_factory(capacity or None) -> plan. Returns (plan, result)."""
+    plan_factory(capacity or None) -> plan. Returns (plan, result). Raises
+    RuntimeError when K exceeds the largest capacity a plan can hold."""
     plan = plan_factory(None)
     for _ in range(max_tries):
         res = run(plan)
         k, over = plan.counters()
         if not over:
             return plan, res
-        cap = int(k * 1.25) + 1024
         plan.close()
-        plan = plan_factory(cap)
+        if k > MAX_PAIRS:
+            raise RuntimeError(f"K = {k} pairs exceeds the plan limit {MAX_PAIRS}")
+        plan = plan_factory(min(int(k * 1.25) + 1024, MAX_PAIRS))
     raise RuntimeError("pair capacity could not be satisfied")

@@ -115,8 +115,11 @@ def tune_multiview(num_pnt, views, tensors, mode="forward", target=None, reps=3,
                 p.backward(xyz, ls, q, op, out_rgb=out, target=target)
             return out
 
-        plan, _ = ensure_capacity(
-            lambda cap: MultiViewPlan(num_pnt, views, ts, th, pair_capacity=cap), step)
+        try:
+            plan, _ = ensure_capacity(
+                lambda cap: MultiViewPlan(num_pnt, views, ts, th, pair_capacity=cap), step)
+        except RuntimeError:
+            continue  # K beyond what one plan can hold at this tile size
         step(plan)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -126,6 +129,8 @@ def tune_multiview(num_pnt, views, tensors, mode="forward", target=None, reps=3,
         res[f"{ts}/{th}"] = (time.perf_counter() - t0) * 1000.0 / reps
         plan.close()
         torch.cuda.empty_cache()
+    if not res:
+        raise RuntimeError("no tile size fits the pair limit of a plan")
     best = min(res, key=res.get)
     ts, th = (int(x) for x in best.split("/"))
     db[sig] = {"best": [ts, th], "ms": res}

@@ -145,6 +145,27 @@ def test_radix_sort_stable(n, bits):
     np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), vals[order])
 
 
+def test_radix_sort_large_repeated():
+    """Thousands of tiles per pass (long look-back chains), the same
+    workspace reused for three sorts of different sizes: every sort must
+    clear what the previous one left."""
+    from del_splat_dlpack import _native
+    L = _native.lib()
+    nmax = 6_000_011
+    wsb = L.dsplat_radix_sort_workspace_size(nmax)
+    ws = torch.full((wsb,), 0xA5, device=DEV, dtype=torch.uint8)  # garbage in the workspace
+    for n, bits in ((nmax, 40), (1_234_567, 31), (nmax - 5, 17)):
+        rng = np.random.default_rng(n)
+        keys = rng.integers(0, 1 << bits, n, dtype=np.uint64)
+        k = torch.from_numpy(keys.view(np.int64).copy()).to(DEV)
+        v = torch.arange(n, device=DEV, dtype=torch.int32)
+        _native.call("dsplat_radix_sort_by_key_u64_u32", torch.cuda.current_stream().cuda_stream,
+                     n, k.data_ptr(), v.data_ptr(), bits, ws.data_ptr(), wsb)
+        order = np.argsort(keys, kind="stable")
+        np.testing.assert_array_equal(k.cpu().numpy().view(np.uint64), keys[order])
+        np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), order.astype(np.uint32))
+
+
 @pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 1_000_001])
 def test_exclusive_scan(n):
     from del_splat_dlpack import _native
