@@ -80,11 +80,13 @@ int launch_bwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
   if (ts == 8) {
     DSPLAT_BWD(8, 64);
   } else if (ts == 16) {
-    if (threads == 64) DSPLAT_BWD(16, 64);
-    else if (threads == 128) DSPLAT_BWD(16, 128);
-    else DSPLAT_BWD(16, 256);
+    if (threads == 128) DSPLAT_BWD(16, 128);
+    else if (threads == 256) DSPLAT_BWD(16, 256);
+    else DSPLAT_BWD(16, 64);
   } else if (ts == 32) {
-    DSPLAT_BWD(32, 256);
+    if (threads == 64) DSPLAT_BWD(32, 64);
+    else if (threads == 128) DSPLAT_BWD(32, 128);
+    else DSPLAT_BWD(32, 256);
   } else {
     return dsplat_set_error(DSPLAT_ERR_ARG, "tile_size must be 8, 16 or 32");
   }

@@ -16,13 +16,16 @@
 // and the workgroup leaves the tile as soon as every pixel has terminated
 // (__syncthreads_and).
 //
-// Adjoint: every pixel walks its list back-to-front from the last entry the
-// forward visited (nwalk) with the transmittance recovered by division from
-// tlast (the transmittance in front of the last contributing splat; dividing
-// by 1 - e of earlier splats is safe because the walk did not stop there, so
-// T stayed above ~1e-3). Per list entry the nine gradient terms of all
-// pixels are summed across each 16-lane row with DPP, across rows and waves
-// with LDS float atomics, and once per (tile, entry) into global memory.
+// State for the adjoint: nwalk = 1 + list position of the last splat that
+// contributed to the pixel (the terminating one if the walk terminated, 0 if
+// none) and tlast = the transmittance in front of that splat.
+//
+// Adjoint: every pixel walks its list back-to-front from position nwalk - 1
+// with the transmittance recovered by division from tlast (dividing by 1 - e
+// of earlier splats is safe because the walk did not stop there, so T stayed
+// above ~1e-3). Per list entry the nine gradient terms of a wave's pixels
+// are summed across the wave with DPP, across waves through per-wave LDS
+// slots, and once per (tile, entry) into global memory.
 #pragma once
 #include "dsplat_common.hpp"
 
@@ -121,7 +124,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     py[k] = float(iy) + 0.5f;
     cr[k] = cg[k] = cbl[k] = asum[k] = 0.f;
     tr[k] = tl[k] = 1.f;
-    walked[k] = num;
+    walked[k] = 0;
     done[k] = !(ix < img_w && iy < img_h);
     live += done[k] ? 0 : 1;
   }
@@ -167,10 +170,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
         cg[k] += q1.w * eo;
         cbl[k] += q2.x * eo;
         tl[k] = in ? tr[k] : tl[k];
+        walked[k] = in ? b + j + 1 : walked[k];
         tr[k] *= 1.f - e;
         asum[k] += eo;
         const bool term = in && asum[k] >= 0.999f;
-        walked[k] = term ? b + j + 1 : walked[k];
         done[k] = done[k] || term;
         live -= term ? 1 : 0;
       }
@@ -221,6 +224,24 @@ __device__ inline float wave_sum_to_last(float v) {
   return v;
 }
 
+// wave_sum_to_last over N values at once, level by level, so consecutive DPP
+// reads of one value are N instructions apart (no wait states between them).
+template <int N>
+__device__ inline void wave_sums_to_last(float (&v)[N]) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0xB1>(v[c]);
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x4E>(v[c]);
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x141>(v[c]);
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x140>(v[c]);
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x142, 0xA>(v[c]);
+#pragma unroll
+  for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x143, 0xC>(v[c]);
+}
+
 // Reduced per list entry: S_x, S_y, S_xx, S_xy, S_yy (moments of dL/dq over
 // the pixel offsets), dL/dopacity, dL/drgb.
 constexpr int kGrad = 9;
@@ -234,19 +255,47 @@ struct GradSink {
   float* d_rgb;     // [gaussians, 3]
 };
 
-// Adjoint of one tile. dL/drgb is read from d_img, or -- when d_img is null
-// -- formed in place as 2 (rgb - target) for the loss sum (rgb - target)^2.
-// kAlpha: whether dL/dalpha is present (d_alpha != null).
+// Pixel -> thread mapping of the adjoint. The TS x TS tile is cut into
+// RS x RS regions (RS = 16 when there are at least as many of them as
+// waves, else 8); each of the W = NT/64 waves owns G of them ("groups"), and
+// in each group every lane a QS x QS pixel quad (QS = RS/8). Box tests per
+// group are wave-uniform, so a wave computes only the groups an entry's box
+// reaches, and its partial sums cover all its groups before one reduction.
+template <int TS, int NT>
+struct GroupMap {
+  static constexpr int W = NT / 64;
+  static constexpr int RS = (TS / 16) * (TS / 16) >= W ? 16 : 8;
+  static constexpr int QS = RS / 8;
+  static constexpr int Q = QS * QS;
+  static constexpr int RXN = TS / RS;
+  static constexpr int G = RXN * RXN / W;
+  static constexpr int PPT = G * Q;
+  static_assert(TS % RS == 0 && G >= 1 && G * W == RXN * RXN, "unsupported tile / workgroup shape");
+  __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+  __device__ static uint32_t rx0(int g) { return ((wave() * G + g) % RXN) * RS; }
+  __device__ static uint32_t ry0(int g) { return ((wave() * G + g) / RXN) * RS; }
+  __device__ static uint32_t lx(int k) {
+    return rx0(k / Q) + (threadIdx.x & 7) * QS + uint32_t((k % Q) % QS);
+  }
+  __device__ static uint32_t ly(int k) {
+    return ry0(k / Q) + ((threadIdx.x & 63) >> 3) * QS + uint32_t((k % Q) / QS);
+  }
+};
+
 // LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
 // template instance declaring its own arrays would double the footprint).
+// acc holds one partial-sum slot per wave, so waves combine by plain stores.
 template <int NT>
 struct BwdLds {
   float4 s0[NT], s1[NT], s2[NT], s3[NT];
   uint32_t spid[NT];
-  float acc[kGrad][NT];
+  float acc[NT / 64][kGrad][NT];
   uint32_t smax;
 };
 
+// Adjoint of one tile. dL/drgb is read from d_img, or -- when d_img is null
+// -- formed in place as 2 (rgb - target) for the loss sum (rgb - target)^2.
+// kAlpha: whether dL/dalpha is present (d_alpha != null).
 template <int TS, int NT, bool kAlpha>
 __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, uint32_t img_w,
                                                 uint32_t img_h, uint32_t end,
@@ -260,31 +309,39 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out,
                                                 BwdLds<NT>& L) {
-  using PM = PixMap<TS, NT>;
-  constexpr int PPT = PM::PPT;
-  const float cx0 = float(tx * TS + PM::rx0()) + 0.5f, cy0 = float(ty * TS + PM::ry0()) + 0.5f;
-  const float cx1 = cx0 + float(PM::RW - 1), cy1 = cy0 + float(PM::RH - 1);
+  using GM = GroupMap<TS, NT>;
+  constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q, W = GM::W;
+  // pixel-centre bounds of each group's region (wave-uniform)
+  float gx0[G], gy0[G], gx1[G], gy1[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    gx0[g] = float(tx * TS + GM::rx0(g)) + 0.5f;
+    gy0[g] = float(ty * TS + GM::ry0(g)) + 0.5f;
+    gx1[g] = gx0[g] + float(GM::RS - 1);
+    gy1[g] = gy0[g] + float(GM::RS - 1);
+  }
   float4* s0 = L.s0;
   float4* s1 = L.s1;
   float4* s2 = L.s2;
   float4* s3 = L.s3;
   uint32_t* spid = L.spid;
-  auto& acc = L.acc;
   uint32_t& smax = L.smax;
+  const uint32_t wv = GM::wave();
+  float(&acc)[kGrad][NT] = L.acc[wv];
   const int lane = threadIdx.x & 63;
   float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
   // gb = g . (colour behind) + ga * (alpha behind): the back accumulators
   // only enter dL/de through this dot product
   float gb[PPT];
+  // nw[k] = nwalk: the pixel's last contributing splat sits at position
+  // nw - 1, and its T is tlast itself (no division)
   uint32_t nw[PPT];
-  bool first[PPT];
   uint32_t my_max = 0;
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
+    const uint32_t ix = tx * TS + GM::lx(k), iy = ty * TS + GM::ly(k);
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
-    first[k] = true;
     gb[k] = 0.f;
     if (ix < img_w && iy < img_h) {
       const size_t ip = size_t(iy) * img_w + ix;
@@ -314,7 +371,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   wmax = __builtin_amdgcn_readfirstlane(wmax);
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
-  atomicMax(&smax, my_max);
+  if (lane == 0) atomicMax(&smax, wmax);
   __syncthreads();
   const uint32_t maxw = smax;
   for (uint32_t hi = maxw; hi > 0;) {
@@ -329,93 +386,114 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       s2[threadIdx.x] = r[2];
       s3[threadIdx.x] = r[3];
       spid[threadIdx.x] = p;
+    }
+    // this wave's partial slot: zero for the entries it skips
+    for (uint32_t j = lane; j < cnt; j += 64) {
 #pragma unroll
-      for (int c = 0; c < kGrad; ++c) acc[c][threadIdx.x] = 0.f;
+      for (int c = 0; c < kGrad; ++c) acc[c][j] = 0.f;
     }
     __syncthreads();
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t it = hi - 1 - j;
       if (it >= wmax) continue;  // beyond every walk of the wave
       const float4 q2 = s2[j], q3 = s3[j];
-      const float4 q0 = s0[j], q1 = s1[j];
-      if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
-      bool inm[PPT];
-      bool any = false;
+      bool hit[G];
+      bool anyg = false;
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        inm[k] = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
-                 py[k] <= q3.x;
-        any = any || inm[k];
+      for (int g = 0; g < G; ++g) {
+        hit[g] = region_hit(q2.y, q2.z, q2.w, q3.x, gx0[g], gy0[g], gx1[g], gy1[g]);
+        anyg = anyg || hit[g];
       }
-      if (__ballot(any) == 0) continue;  // no pixel of the wave takes this entry
+      if (!anyg) continue;  // the box reaches none of the wave's regions
+      const float4 q0 = s0[j], q1 = s1[j];
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
-      // Branch-free per pixel: a splat that misses the pixel gets gk = e = 0,
-      // which adds exact zeros and keeps T (T * rcp(1) = T).
+      bool touched = false;
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        const bool in = inm[k];
-        const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-        const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
-        const float gk = in ? fast_exp(-0.5f * q) : 0.f;
-        const float e = gk * q1.y;
-        const float ome = 1.f - e;
-        const float inv = __builtin_amdgcn_rcpf(ome);
-        tr[k] = first[k] ? tr[k] : tr[k] * inv;
-        first[k] = first[k] && !in;
-        const float t = tr[k];
-        // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)
-        float gc = g0[k] * q1.z + g1[k] * q1.w + g2[k] * q2.x;
-        if (kAlpha) gc += ga[k];
-        const float dl_de = t * gc - gb[k] * inv;
-        const float et = e * t;
-        v[6] += g0[k] * et;
-        v[7] += g1[k] * et;
-        v[8] += g2[k] * et;
-        gb[k] += gc * et;
-        v[5] += dl_de * gk;
-        // moments of dL/dq: the mean and conic gradients of the entry are
-        // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
-        const float dl_dq = dl_de * e * -0.5f;
-        const float a = dl_dq * dx, bq = dl_dq * dy;
-        v[0] += a;
-        v[1] += bq;
-        v[2] += a * dx;
-        v[3] += a * dy;
-        v[4] += bq * dy;
-      }
+      for (int g = 0; g < G; ++g) {
+        if (!hit[g]) continue;  // wave-uniform
+        bool inm[Q];
+        bool any = false;
 #pragma unroll
-      for (int c = 0; c < kGrad; ++c) {
-        const float r = wave_sum_to_last(v[c]);
-        if (lane == 63) {
-          if (NT == 64) acc[c][j] = r;
-          else atomicAdd(&acc[c][j], r);
+        for (int qq = 0; qq < Q; ++qq) {
+          const int k = g * Q + qq;
+          inm[qq] = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
+                    py[k] <= q3.x;
+          any = any || inm[qq];
         }
+        if (__ballot(any) == 0) continue;  // no pixel of the group takes this entry
+        touched = true;
+        // Branch-free per pixel: a splat that misses the pixel gets gk = e = 0,
+        // which adds exact zeros and keeps T (T * rcp(1) = T).
+#pragma unroll
+        for (int qq = 0; qq < Q; ++qq) {
+          const int k = g * Q + qq;
+          const bool in = inm[qq];
+          const float dx = px[k] - q0.x, dy = py[k] - q0.y;
+          const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
+          // exp(-inf) = 0: a select on the argument keeps the exp unpredicated
+          const float gk = fast_exp(in ? -0.5f * q : -__builtin_inff());
+          const float e = gk * q1.y;
+          const float ome = 1.f - e;
+          const float inv = __builtin_amdgcn_rcpf(ome);
+          tr[k] = it + 1u == nw[k] ? tr[k] : tr[k] * inv;
+          const float t = tr[k];
+          // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)
+          float gc = g0[k] * q1.z + g1[k] * q1.w + g2[k] * q2.x;
+          if (kAlpha) gc += ga[k];
+          const float dl_de = t * gc - gb[k] * inv;
+          const float et = e * t;
+          v[6] += g0[k] * et;
+          v[7] += g1[k] * et;
+          v[8] += g2[k] * et;
+          gb[k] += gc * et;
+          v[5] += dl_de * gk;
+          // moments of dL/dq: the mean and conic gradients of the entry are
+          // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
+          const float dl_dq = dl_de * e * -0.5f;
+          const float a = dl_dq * dx, bq = dl_dq * dy;
+          v[0] += a;
+          v[1] += bq;
+          v[2] += a * dx;
+          v[3] += a * dy;
+          v[4] += bq * dy;
+        }
+      }
+      if (!touched) continue;
+      wave_sums_to_last(v);
+      if (lane == 63) {
+#pragma unroll
+        for (int c = 0; c < kGrad; ++c) acc[c][j] = v[c];
       }
     }
     __syncthreads();
     if (threadIdx.x < cnt) {
-      const uint32_t p = spid[threadIdx.x];
+      const uint32_t t = threadIdx.x;
+      float sum[kGrad];
+#pragma unroll
+      for (int c = 0; c < kGrad; ++c) {
+        sum[c] = L.acc[0][c][t];
+#pragma unroll
+        for (int w = 1; w < W; ++w) sum[c] += L.acc[w][c][t];
+      }
+      const uint32_t p = spid[t];
       const uint32_t g = p - gauss_base;
-      const float sx = acc[0][threadIdx.x], sy = acc[1][threadIdx.x];
-      const float ca = s0[threadIdx.x].z, cb = s0[threadIdx.x].w, cc = s1[threadIdx.x].x;
+      const float ca = s0[t].z, cb = s0[t].w, cc = s1[t].x;
       // d q / d mean = -2 C d  ->  dL/dmean = -2 C (S_x, S_y)
-      const float a0 = -2.f * (ca * sx + cb * sy), a1 = -2.f * (cb * sx + cc * sy);
+      const float a0 = -2.f * (ca * sum[0] + cb * sum[1]), a1 = -2.f * (cb * sum[0] + cc * sum[1]);
       if (a0 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p)], a0);
       if (a1 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p) + 1], a1);
       // d q / d(a, b, c) = (dx^2, 2 dx dy, dy^2)
-      const float dc[3] = {acc[2][threadIdx.x], 2.f * acc[3][threadIdx.x], acc[4][threadIdx.x]};
+      const float dc[3] = {sum[2], 2.f * sum[3], sum[4]};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (dc[c] != 0.f) atomicAdd(&out.d_conic[3 * size_t(p) + c], dc[c]);
       }
-      const float ao = acc[5][threadIdx.x];
-      if (ao != 0.f) atomicAdd(&out.d_op[g], ao);
+      if (sum[5] != 0.f) atomicAdd(&out.d_op[g], sum[5]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const float a = acc[6 + c][threadIdx.x];
-        if (a != 0.f) atomicAdd(&out.d_rgb[3 * size_t(g) + c], a);
+        if (sum[6 + c] != 0.f) atomicAdd(&out.d_rgb[3 * size_t(g) + c], sum[6 + c]);
       }
     }
     __syncthreads();

@@ -44,14 +44,14 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
     threads_bwd), {config: ms})."""
     from .SplatGauss import torch as SG
 
-    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd_v2")
+    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd_v3")
     db = _load() if use_cache else {}
     if sig in db:
         e = db[sig]
         return tuple(e["best"]), dict(e["ms"])
     xyz, ls, q, op, rgb = tensors
     fwd_opts = {8: (64,), 16: (64, 128, 256), 32: (256, 512, 1024)}
-    bwd_opts = {8: (64,), 16: (64, 128, 256), 32: (256,)}
+    bwd_opts = {8: (64,), 16: (64, 128, 256), 32: (64, 128, 256)}
 
     def timed(fn):
         fn()

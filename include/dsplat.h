@@ -128,8 +128,9 @@ int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xy
 
 /* Tiled raster over packed records, with the state the adjoint needs:
  * pix2rgb [H,W,3], pix2alpha [H,W] (the reference's alpha_sum), pix2nwalk
- * u32 [H,W] (tile-list entries walked, including the one that terminated),
- * pix2tlast [H,W] (transmittance in front of the last contributing splat).
+ * u32 [H,W] (1 + walk position of the last splat that contributed to the
+ * pixel -- the terminating one if the walk terminated --, 0 if none),
+ * pix2tlast [H,W] (transmittance in front of that splat, 1 if none).
  * pix2alpha / pix2nwalk / pix2tlast may be null. `threads` selects the
  * workgroup size per tile (0 = default; ts 16: 64/128/256, ts 32:
  * 256/512/1024; ts 8: always 64). */
@@ -142,7 +143,9 @@ int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
 /* Adjoint of dsplat_gauss_rasterize_fwd. Walks every pixel's list
  * back-to-front from pix2tlast. ADDS into d_pnt2pixxy [N,2], d_pnt2conic
  * [N,3], d_pnt2opacity [N], d_pnt2rgb [N,3]. d_pix2alpha may be null.
- * `threads` as for dsplat_gauss_rasterize_fwd (ts 32: 256 only). */
+ * `threads` selects the workgroup size per tile (0 = default; ts 16:
+ * 64/128/256, default 64; ts 32: 64/128/256, default 256; ts 8: always
+ * 64). */
 int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                const uint32_t* tile2idx, const uint32_t* idx2pnt,

@@ -299,8 +299,9 @@ uint64_t tile2pnt(uint32_t n, const float* boxes, const float* pixxydepth,
 // Tiled front-to-back raster: splat_gauss.cu:56-109
 // ---------------------------------------------------------------------------
 // Extra outputs beyond the reference's RGB: alpha (= the reference's
-// alpha_sum), nwalk (list entries walked, including the terminating one) and
-// tlast (transmittance in front of the last contributing splat, 1 if none).
+// alpha_sum), nwalk (1 + list position of the last splat that contributed
+// to the pixel -- the terminating one if the walk terminated --, 0 if none)
+// and tlast (transmittance in front of that splat, 1 if none).
 // If frozen_nwalk != null the walk length per pixel is taken from it and the
 // termination test is skipped (the function the adjoint differentiates).
 template <typename T>
@@ -318,7 +319,7 @@ void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const 
       const uint32_t i0 = tile2idx[tile], num = tile2idx[tile + 1] - i0;
       T c[3] = {0, 0, 0};
       T alpha_sum = 0, alpha_occu = 1, tlast = 1;
-      uint32_t walked = num;
+      uint32_t walked = 0;
       const uint32_t lim = frozen_nwalk ? frozen_nwalk[ipix] : num;
       for (uint32_t it = 0; it < lim; ++it) {
         const uint32_t p = idx2pnt[i0 + num - 1 - it];
@@ -335,10 +336,11 @@ void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const 
         c[1] += rgb[3 * p + 1] * eo;
         c[2] += rgb[3 * p + 2] * eo;
         tlast = alpha_occu;
+        walked = it + 1;
         alpha_occu *= T(1) - e;
         alpha_sum += eo;
         // the reference compares the float sum against the double 0.999
-        if (!frozen_nwalk && double(alpha_sum) > 0.999) { walked = it + 1; break; }
+        if (!frozen_nwalk && double(alpha_sum) > 0.999) break;
       }
       if (frozen_nwalk) walked = lim;
       for (int k = 0; k < 3; ++k) pix2rgb[3 * ipix + k] = c[k];

@@ -277,3 +277,33 @@ def test_backward_matches_oracle_adjoint(oracle, ts, with_alpha):
         err = max_rel(g.cpu().numpy(), o)
         print(f"{name}: max rel err {err:.2e}")
         assert err <= 2e-3, (name, err)
+
+
+@pytest.mark.parametrize("ts,threads", [(16, 64), (16, 128), (16, 256), (32, 64), (32, 128),
+                                        (32, 256), (8, 64)])
+@pytest.mark.parametrize("with_alpha", [False, True])
+def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
+    """The adjoint raster at every workgroup shape (pixel groups per wave,
+    per-wave partial slots) against the oracle adjoint on the same lists; the
+    image is not a multiple of the tile size."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (171, 133)
+    s, m = small_scene(4000, 40 + ts, shape)
+    op, rgb = _t(s["opacity"]), _t(s["rgb"])
+    pxd, conic, aabb, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op,
+                                              rgb, _t(m), shape, log_scale=False)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
+    rng = np.random.default_rng(ts + threads)
+    d_img = rng.standard_normal((shape[1], shape[0], 3)).astype(np.float32)
+    d_alpha = rng.standard_normal((shape[1], shape[0])).astype(np.float32) if with_alpha else None
+    g = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast, _t(d_img),
+                         None if d_alpha is None else _t(d_alpha), threads)
+    o = oracle.gauss_rasterize_bwd(shape, pxd.cpu().numpy(), conic.cpu().numpy(),
+                                   aabb.cpu().numpy(), s["opacity"], s["rgb"], ts, u32(t2i),
+                                   u32(i2p), nwalk.cpu().numpy().view(np.uint32),
+                                   tlast.cpu().numpy(), d_img, d_alpha)
+    for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
+        err = max_rel(a.cpu().numpy(), b)
+        print(f"ts={ts} threads={threads} alpha={with_alpha} {name}: max rel err {err:.2e}")
+        assert err <= 2e-3, (name, err)
