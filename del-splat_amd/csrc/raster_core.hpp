@@ -62,33 +62,52 @@ struct ArrayLoader {
 
 __device__ inline float fast_exp(float x) { return __expf(x); }
 
-// Pixel -> thread mapping of a TS x TS tile worked by NT threads with PPT =
-// TS^2/NT pixels each: every wave owns a compact RW x RH region (8x8, 16x8
-// or 16x16) and every lane a QX x QY pixel quad in it, so box tests are
-// coherent within a wave and a whole wave can skip an entry whose box misses
-// its region.
+// The splat weight's pieces in one fixed evaluation order (contraction off;
+// fused only where fmaf is written), shared by the forward and the adjoint,
+// so every workgroup shape produces bit-identical pixels and the adjoint
+// recomputes exactly the forward's e.
+// q = d^T C d for d = (dx, dy), C = [[a, b], [b, c]].
+__device__ __forceinline__ float conic_q(float a, float b, float c, float dx, float dy) {
+#pragma clang fp contract(off)
+  return fmaf(a * dx, dx, fmaf((b + b) * dx, dy, c * dy * dy));
+}
+// exp(-q/2) where in, else 0 (exp(-inf) = 0: a select on the argument
+// keeps the exp unpredicated)
+__device__ __forceinline__ float splat_kernel(bool in, float q) {
+#pragma clang fp contract(off)
+  return fast_exp(in ? -0.5f * q : -__builtin_inff());
+}
+
+// Pixel -> thread mapping of a tile (forward and adjoint). The TS x TS tile
+// is cut into RS x RS regions (RS = 16 when there are at least as many of
+// them as waves, else 8); each of the W = NT/64 waves owns G of them
+// ("groups"), and in each group every lane a QS x QS pixel quad (QS = RS/8).
+// Box tests per group are wave-uniform, so a wave computes only the groups
+// an entry's box reaches (and in the adjoint its partial sums cover all its
+// groups before one reduction).
 template <int TS, int NT>
-struct PixMap {
-  static constexpr int PPT = TS * TS / NT;
-  static constexpr int QX = PPT >= 2 ? 2 : 1, QY = PPT >= 4 ? 2 : 1;
-  static constexpr int RW = 8 * QX, RH = 8 * QY;
-  static constexpr int RX = TS / RW;
-  static_assert(PPT == QX * QY && TS % RW == 0 && (NT / 64) * RW * RH == TS * TS,
-                "unsupported tile / workgroup shape");
-  // wave-uniform region origin (in tile-local pixels)
+struct GroupMap {
+  static constexpr int W = NT / 64;
+  static constexpr int RS = (TS / 16) * (TS / 16) >= W ? 16 : 8;
+  static constexpr int QS = RS / 8;
+  static constexpr int Q = QS * QS;
+  static constexpr int RXN = TS / RS;
+  static constexpr int G = RXN * RXN / W;
+  static constexpr int PPT = G * Q;
+  static_assert(TS % RS == 0 && G >= 1 && G * W == RXN * RXN, "unsupported tile / workgroup shape");
   __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-  __device__ static uint32_t rx0() { return (wave() % RX) * RW; }
-  __device__ static uint32_t ry0() { return (wave() / RX) * RH; }
+  __device__ static uint32_t rx0(int g) { return ((wave() * G + g) % RXN) * RS; }
+  __device__ static uint32_t ry0(int g) { return ((wave() * G + g) / RXN) * RS; }
   __device__ static uint32_t lx(int k) {
-    return rx0() + (threadIdx.x & 7) * QX + uint32_t(k % QX);
+    return rx0(k / Q) + (threadIdx.x & 7) * QS + uint32_t((k % Q) % QS);
   }
   __device__ static uint32_t ly(int k) {
-    return ry0() + ((threadIdx.x & 63) >> 3) * QY + uint32_t(k / QX);
+    return ry0(k / Q) + ((threadIdx.x & 63) >> 3) * QS + uint32_t((k % Q) / QS);
   }
 };
 
-// Whether the box (x0, y0, x1, y1) can contain a pixel centre of the
-// wave's region [cx0, cx0 + RW - 1] x [cy0, cy0 + RH - 1] (centres).
+// Whether the box (x0, y0, x1, y1) can contain a pixel centre of the region
+// [cx0, cx1] x [cy0, cy1] (centres).
 __device__ inline bool region_hit(float bx0, float by0, float bx1, float by1, float cx0,
                                   float cy0, float cx1, float cy1) {
   return !(bx1 < cx0 || bx0 > cx1 || by1 < cy0 || by0 > cy1);
@@ -106,20 +125,27 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                                                 float* __restrict__ pix2alpha,
                                                 uint32_t* __restrict__ pix2nwalk,
                                                 float* __restrict__ pix2tlast) {
-  using PM = PixMap<TS, NT>;
-  constexpr int PPT = PM::PPT;
+#pragma clang fp contract(off)
+  using GM = GroupMap<TS, NT>;
+  constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q;
   __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
   const uint32_t num = end - start;
   float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
   uint32_t walked[PPT];
   bool done[PPT];
   int live = 0;
-  // pixel-centre bounds of the wave's region
-  const float cx0 = float(tx * TS + PM::rx0()) + 0.5f, cy0 = float(ty * TS + PM::ry0()) + 0.5f;
-  const float cx1 = cx0 + float(PM::RW - 1), cy1 = cy0 + float(PM::RH - 1);
+  // pixel-centre bounds of each group's region (wave-uniform)
+  float gx0[G], gy0[G], gx1[G], gy1[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    gx0[g] = float(tx * TS + GM::rx0(g)) + 0.5f;
+    gy0[g] = float(ty * TS + GM::ry0(g)) + 0.5f;
+    gx1[g] = gx0[g] + float(GM::RS - 1);
+    gy1[g] = gy0[g] + float(GM::RS - 1);
+  }
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
+    const uint32_t ix = tx * TS + GM::lx(k), iy = ty * TS + GM::ly(k);
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
     cr[k] = cg[k] = cbl[k] = asum[k] = 0.f;
@@ -142,47 +168,60 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     const uint32_t cnt = min(uint32_t(NT), num - b);
     // Branch-free per pixel: a splat that misses the pixel (or a finished
     // pixel) gets e = 0, which leaves every accumulator bit-identical
-    // (x + 0 = x, T * 1 = T); only whole-wave skips branch.
+    // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
     for (uint32_t j = 0; j < cnt; ++j) {
       if (__ballot(live > 0) == 0) break;  // every pixel of the wave is done
       const float4 q2 = s2[j];
       const float4 q3 = s3[j];
+      bool hit[G];
+      bool anyg = false;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        hit[g] = region_hit(q2.y, q2.z, q2.w, q3.x, gx0[g], gy0[g], gx1[g], gy1[g]);
+        anyg = anyg || hit[g];
+      }
+      if (!anyg) continue;  // the box reaches none of the wave's regions
       const float4 q0 = s0[j];
       const float4 q1 = s1[j];
-      if (!region_hit(q2.y, q2.z, q2.w, q3.x, cx0, cy0, cx1, cy1)) continue;  // whole wave
-      bool inm[PPT];
-      bool any = false;
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        // inclusive box test (aabb2::is_include_point)
-        inm[k] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
-        any = any || inm[k];
-      }
-      if (__ballot(any) == 0) continue;  // no live pixel of the wave is inside the box
+      for (int g = 0; g < G; ++g) {
+        if (!hit[g]) continue;  // wave-uniform
+        bool inm[Q];
+        bool any = false;
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        const bool in = inm[k];
-        const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-        const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
-        const float e = in ? fast_exp(-0.5f * q) * q1.y : 0.f;
-        const float eo = tr[k] * e;
-        cr[k] += q1.z * eo;
-        cg[k] += q1.w * eo;
-        cbl[k] += q2.x * eo;
-        tl[k] = in ? tr[k] : tl[k];
-        walked[k] = in ? b + j + 1 : walked[k];
-        tr[k] *= 1.f - e;
-        asum[k] += eo;
-        const bool term = in && asum[k] >= 0.999f;
-        done[k] = done[k] || term;
-        live -= term ? 1 : 0;
+        for (int qq = 0; qq < Q; ++qq) {
+          const int k = g * Q + qq;
+          // inclusive box test (aabb2::is_include_point)
+          inm[qq] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
+          any = any || inm[qq];
+        }
+        if (__ballot(any) == 0) continue;  // no live pixel of the group is inside the box
+#pragma unroll
+        for (int qq = 0; qq < Q; ++qq) {
+          const int k = g * Q + qq;
+          const bool in = inm[qq];
+          const float dx = px[k] - q0.x, dy = py[k] - q0.y;
+          const float q = conic_q(q0.z, q0.w, q1.x, dx, dy);
+          const float e = splat_kernel(in, q) * q1.y;
+          const float eo = tr[k] * e;
+          cr[k] = fmaf(q1.z, eo, cr[k]);
+          cg[k] = fmaf(q1.w, eo, cg[k]);
+          cbl[k] = fmaf(q2.x, eo, cbl[k]);
+          tl[k] = in ? tr[k] : tl[k];
+          walked[k] = in ? b + j + 1 : walked[k];
+          tr[k] = tr[k] * (1.f - e);
+          asum[k] = asum[k] + eo;
+          const bool term = in && asum[k] >= 0.999f;
+          done[k] = done[k] || term;
+          live -= term ? 1 : 0;
+        }
       }
     }
     if (__syncthreads_and(live == 0)) break;
   }
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
-    const uint32_t ix = tx * TS + PM::lx(k), iy = ty * TS + PM::ly(k);
+    const uint32_t ix = tx * TS + GM::lx(k), iy = ty * TS + GM::ly(k);
     if (ix >= img_w || iy >= img_h) continue;
     const size_t ip = size_t(iy) * img_w + ix;
     pix2rgb[3 * ip + 0] = cr[k];
@@ -255,33 +294,6 @@ struct GradSink {
   float* d_rgb;     // [gaussians, 3]
 };
 
-// Pixel -> thread mapping of the adjoint. The TS x TS tile is cut into
-// RS x RS regions (RS = 16 when there are at least as many of them as
-// waves, else 8); each of the W = NT/64 waves owns G of them ("groups"), and
-// in each group every lane a QS x QS pixel quad (QS = RS/8). Box tests per
-// group are wave-uniform, so a wave computes only the groups an entry's box
-// reaches, and its partial sums cover all its groups before one reduction.
-template <int TS, int NT>
-struct GroupMap {
-  static constexpr int W = NT / 64;
-  static constexpr int RS = (TS / 16) * (TS / 16) >= W ? 16 : 8;
-  static constexpr int QS = RS / 8;
-  static constexpr int Q = QS * QS;
-  static constexpr int RXN = TS / RS;
-  static constexpr int G = RXN * RXN / W;
-  static constexpr int PPT = G * Q;
-  static_assert(TS % RS == 0 && G >= 1 && G * W == RXN * RXN, "unsupported tile / workgroup shape");
-  __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-  __device__ static uint32_t rx0(int g) { return ((wave() * G + g) % RXN) * RS; }
-  __device__ static uint32_t ry0(int g) { return ((wave() * G + g) / RXN) * RS; }
-  __device__ static uint32_t lx(int k) {
-    return rx0(k / Q) + (threadIdx.x & 7) * QS + uint32_t((k % Q) % QS);
-  }
-  __device__ static uint32_t ly(int k) {
-    return ry0(k / Q) + ((threadIdx.x & 63) >> 3) * QS + uint32_t((k % Q) / QS);
-  }
-};
-
 // LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
 // template instance declaring its own arrays would double the footprint).
 // acc holds one partial-sum slot per wave, so waves combine by plain stores.
@@ -309,6 +321,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out,
                                                 BwdLds<NT>& L) {
+#pragma clang fp contract(off)
   using GM = GroupMap<TS, NT>;
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q, W = GM::W;
   // pixel-centre bounds of each group's region (wave-uniform)
@@ -431,33 +444,32 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           const int k = g * Q + qq;
           const bool in = inm[qq];
           const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-          const float q = q0.z * dx * dx + q0.w * (dx * dy + dy * dx) + q1.x * dy * dy;
-          // exp(-inf) = 0: a select on the argument keeps the exp unpredicated
-          const float gk = fast_exp(in ? -0.5f * q : -__builtin_inff());
+          const float q = conic_q(q0.z, q0.w, q1.x, dx, dy);
+          const float gk = splat_kernel(in, q);
           const float e = gk * q1.y;
           const float ome = 1.f - e;
           const float inv = __builtin_amdgcn_rcpf(ome);
           tr[k] = it + 1u == nw[k] ? tr[k] : tr[k] * inv;
           const float t = tr[k];
           // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)
-          float gc = g0[k] * q1.z + g1[k] * q1.w + g2[k] * q2.x;
-          if (kAlpha) gc += ga[k];
-          const float dl_de = t * gc - gb[k] * inv;
+          float gc = fmaf(g0[k], q1.z, fmaf(g1[k], q1.w, g2[k] * q2.x));
+          if (kAlpha) gc = gc + ga[k];
+          const float dl_de = fmaf(t, gc, -(gb[k] * inv));
           const float et = e * t;
-          v[6] += g0[k] * et;
-          v[7] += g1[k] * et;
-          v[8] += g2[k] * et;
-          gb[k] += gc * et;
-          v[5] += dl_de * gk;
+          v[6] = fmaf(g0[k], et, v[6]);
+          v[7] = fmaf(g1[k], et, v[7]);
+          v[8] = fmaf(g2[k], et, v[8]);
+          gb[k] = fmaf(gc, et, gb[k]);
+          v[5] = fmaf(dl_de, gk, v[5]);
           // moments of dL/dq: the mean and conic gradients of the entry are
           // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
-          const float dl_dq = dl_de * e * -0.5f;
+          const float dl_dq = (dl_de * e) * -0.5f;
           const float a = dl_dq * dx, bq = dl_dq * dy;
-          v[0] += a;
-          v[1] += bq;
-          v[2] += a * dx;
-          v[3] += a * dy;
-          v[4] += bq * dy;
+          v[0] = v[0] + a;
+          v[1] = v[1] + bq;
+          v[2] = fmaf(a, dx, v[2]);
+          v[3] = fmaf(a, dy, v[3]);
+          v[4] = fmaf(bq, dy, v[4]);
         }
       }
       if (!touched) continue;

@@ -215,7 +215,7 @@ def test_raster_parity(oracle, ts):
     # packed-record path with the adjoint's state
     _, _, _, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op, rgb,
                                      _t(m), shape, log_scale=False)
-    for thr in ([0, 64, 128] if ts == 16 else [0, 512, 1024] if ts == 32 else [0]):
+    for thr in ([0, 64, 128, 256] if ts == 16 else [0, 128, 256, 512, 1024] if ts == 32 else [0]):
         g_img, g_alpha, g_nwalk, g_tlast = (t.cpu().numpy() for t in SG.rasterize_fwd(
             shape, ts, splat2, t2i, i2p, thr))
         flips = _check_pixels(g_img, g_alpha, g_nwalk.view(np.uint32), o_img, o_alpha, o_nwalk)
