@@ -33,44 +33,59 @@ struct CircleBoxes {
   }
 };
 
+// cnt[i] = tiles of Gaussian i, dkey[i] = its depth key (dkey may be null);
+// partial[block] = the workgroup's sum of cnt.
 template <typename Boxes>
 __global__ void __launch_bounds__(256)
 bin_count_kernel(uint32_t n, Boxes boxes, const float* __restrict__ pixxydepth, uint32_t ts,
                  uint32_t tw, uint32_t th, uint32_t* __restrict__ cnt,
-                 uint32_t* __restrict__ dkey) {
+                 uint32_t* __restrict__ dkey, uint32_t* __restrict__ partial) {
+  __shared__ uint32_t wsum[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 b = boxes(i);
-  const TileRect r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
-  cnt[i] = uint32_t(r.count());
-  if (dkey) dkey[i] = depth_key(pixxydepth[3 * i + 2]);
+  uint32_t v = 0;
+  if (i < n) {
+    const float4 b = boxes(i);
+    const TileRect r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
+    v = uint32_t(r.count());
+    cnt[i] = v;
+    if (dkey) dkey[i] = depth_key(pixxydepth[3 * i + 2]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += uint32_t(__shfl_xor(int(v), off, 64));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// cnt_sorted[j] = cnt[order[j]]
+// *d_k = sum of the nb partial sums (one workgroup).
 __global__ void __launch_bounds__(256)
-gather_u32_kernel(uint32_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ src,
-                  uint32_t* __restrict__ dst) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) dst[j] = src[order[j]];
-}
-
-// n_eff = K if K <= capacity else 0 (overflow: downstream kernels see nothing)
-__global__ void clamp_count_kernel(const uint32_t* __restrict__ d_k, uint32_t capacity,
-                                   uint32_t* __restrict__ n_eff) {
-  const uint32_t k = *d_k;
-  *n_eff = k <= capacity ? k : 0u;
+sum_partials_kernel(uint32_t nb, const uint32_t* __restrict__ partial, uint32_t* __restrict__ d_k) {
+  __shared__ uint32_t wsum[4];
+  uint32_t v = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += 256) v += partial[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += uint32_t(__shfl_xor(int(v), off, 64));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *d_k = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // Pairs of the j-th Gaussian in depth order at off[j]: tile ids ix-outer,
-// iy-inner (tile_acceleration.h:29-50), value = Gaussian index.
+// iy-inner (tile_acceleration.h:29-50), value = Gaussian index. n_eff = K if
+// K <= capacity else 0 (overflow: nothing is written and the later phases
+// see no pairs); workgroup 0 stores it for them.
 template <typename Boxes>
 __global__ void __launch_bounds__(256)
 bin_fill_kernel(uint32_t n, Boxes boxes, const uint32_t* __restrict__ order,
-                const uint32_t* __restrict__ off, const uint32_t* __restrict__ n_eff,
-                uint32_t ts, uint32_t tw, uint32_t th, uint32_t* __restrict__ tile_of_pair,
+                const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_k,
+                uint32_t capacity, uint32_t* __restrict__ n_eff_out, uint32_t ts, uint32_t tw,
+                uint32_t th, uint32_t* __restrict__ tile_of_pair,
                 uint32_t* __restrict__ pnt_of_pair) {
   __shared__ ExpandLds lds[4];
-  if (*n_eff == 0) return;  // uniform: overflow, nothing is written
+  const uint32_t k = *d_k;
+  const uint32_t n_eff = k <= capacity ? k : 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *n_eff_out = n_eff;
+  if (n_eff == 0) return;  // uniform
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < n;
   uint32_t g = 0, o = 0;
@@ -112,25 +127,31 @@ inline uint32_t bit_width(uint32_t v) {
   return b;
 }
 
-struct BinLayout {
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Workspace of the per-Gaussian phases (count, order), sized by N, and of
+// the per-pair phase (fill, tile sort), sized by the pair capacity.
+struct PointLayout {
   uint32_t* cnt;       // [N]
   uint32_t* dkey;      // [N]
-  uint32_t* order;     // [N]
+  uint32_t* order;     // [N] Gaussians in depth order
   void* dsort_ws;      // depth sort
   size_t dsort_bytes;
-  uint32_t* off;       // [N+1]
+  uint32_t* off;       // [N + 1] pair offsets in depth order
   uint32_t* scan_ws;
-  uint32_t* tilekey;   // [cap]
-  void* tsort_ws;      // tile sort
-  size_t tsort_bytes;
   uint32_t* misc;      // [4]: n_eff
   size_t total;
 };
 
-inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+struct PairLayout {
+  uint32_t* tilekey;   // [cap]
+  void* tsort_ws;      // tile sort
+  size_t tsort_bytes;
+  size_t total;
+};
 
-inline BinLayout bin_layout(void* base, uint32_t n, uint32_t cap) {
-  BinLayout L{};
+inline PointLayout point_layout(void* base, uint32_t n) {
+  PointLayout L{};
   char* p = static_cast<char*>(base);
   size_t o = 0;
   auto take = [&](size_t bytes) { char* q = p ? p + o : nullptr; o += align256(bytes); return q; };
@@ -141,79 +162,102 @@ inline BinLayout bin_layout(void* base, uint32_t n, uint32_t cap) {
   L.dsort_ws = take(L.dsort_bytes);
   L.off = reinterpret_cast<uint32_t*>(take((size_t(n) + 1) * 4));
   L.scan_ws = reinterpret_cast<uint32_t*>(take(scan_workspace_bytes(n)));
-  L.tilekey = reinterpret_cast<uint32_t*>(take(size_t(cap) * 4));
-  L.tsort_bytes = sort_workspace_bytes<uint32_t>(cap);
-  L.tsort_ws = take(L.tsort_bytes);
   L.misc = reinterpret_cast<uint32_t*>(take(16));
   L.total = o;
   return L;
 }
 
+inline PairLayout pair_layout(void* base, uint32_t cap) {
+  PairLayout L{};
+  char* p = static_cast<char*>(base);
+  size_t o = 0;
+  auto take = [&](size_t bytes) { char* q = p ? p + o : nullptr; o += align256(bytes); return q; };
+  L.tilekey = reinterpret_cast<uint32_t*>(take(size_t(cap) * 4));
+  L.tsort_bytes = sort_workspace_bytes<uint32_t>(cap);
+  L.tsort_ws = take(L.tsort_bytes);
+  L.total = o;
+  return L;
+}
+
+// Phase 1: per-Gaussian tile counts and depth keys (none when pixxydepth is
+// null: count only); K = sum of the counts into *d_num_idx.
 template <typename Boxes>
-int count_pairs(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, uint32_t th,
-                uint32_t* cnt, uint32_t* scan_ws, uint32_t* d_num_idx) {
+int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
+              uint32_t tw, uint32_t th, const PointLayout& L, uint32_t* d_num_idx) {
   if (n == 0) {
     set_u32_one<<<1, 1, 0, s>>>(d_num_idx, 0u);
-    DSPLAT_HIP(hipGetLastError());
-    return DSPLAT_OK;
+  } else {
+    // per-workgroup sums go to off[] (rewritten by the order phase)
+    const uint32_t nb = div_up(n, 256);
+    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt,
+                                               pixxydepth ? L.dkey : nullptr, L.off);
+    sum_partials_kernel<<<1, 256, 0, s>>>(nb, L.off, d_num_idx);
   }
-  bin_count_kernel<Boxes><<<div_up(n, 256), 256, 0, s>>>(n, boxes, nullptr, ts, tw, th, cnt,
-                                                         nullptr);
-  // reduce through the scan (out == in is allowed; cnt is scratch here)
-  int rc = exclusive_scan(s, n, nullptr, cnt, cnt, scan_ws, d_num_idx);
-  if (rc) return rc;
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }
 
+// Phase 2: stable depth order of the Gaussians (values generated as iota by
+// the first pass; the last pass also gathers the counts into depth order),
+// then the exclusive scan of the counts = each Gaussian's first pair slot.
+inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L) {
+  if (n == 0) return DSPLAT_OK;
+  bool in_alt = false;
+  uint32_t *kalt = nullptr, *valt = nullptr;
+  int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
+                                      L.dsort_bytes, &in_alt, &kalt, &valt, true, L.cnt, L.off);
+  if (rc) return rc;
+  if (in_alt)  // 31 key bits take 4 passes: the result is in place
+    return dsplat_set_error(DSPLAT_ERR_ARG, "binning: depth order in the alternate buffer");
+  return exclusive_scan(s, n, nullptr, L.off, L.off, L.scan_ws, nullptr);
+}
+
+// Phase 3: pairs in depth order, stable sort by tile id, tile ranges. Writes
+// nothing into idx2pnt (and empty ranges) when K > capacity.
 template <typename Boxes>
-int bin_and_sort(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
-                 uint32_t tw, uint32_t th, uint32_t* tile2idx, uint32_t* idx2pnt,
-                 uint32_t capacity, void* ws, size_t ws_bytes, uint32_t* d_num_idx) {
+int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, uint32_t th,
+             const PointLayout& L, const uint32_t* d_num_idx, uint32_t* tile2idx,
+             uint32_t* idx2pnt, uint32_t capacity, const PairLayout& Q) {
   const uint32_t nt = tw * th;
-  BinLayout L = bin_layout(ws, n, capacity);
-  if (ws_bytes < L.total)
-    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "binning: workspace too small");
   if (n == 0) {
-    set_u32_one<<<1, 1, 0, s>>>(d_num_idx, 0u);
     set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
   } else {
-    const uint32_t nb = div_up(n, 256);
-    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt, L.dkey);
-    bool in_alt = false;
-    uint32_t *kalt = nullptr, *valt = nullptr;
-    // values: Gaussian indices (iota), generated by the first pass
-    int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
-                                        L.dsort_bytes, &in_alt, &kalt, &valt, true);
-    if (rc) return rc;
-    const uint32_t* order = in_alt ? valt : L.order;
-    gather_u32_kernel<<<nb, 256, 0, s>>>(n, order, L.cnt, L.off);
-    rc = exclusive_scan(s, n, nullptr, L.off, L.off, L.scan_ws, d_num_idx);
-    if (rc) return rc;
-    clamp_count_kernel<<<1, 1, 0, s>>>(d_num_idx, capacity, L.misc);
-    bin_fill_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, order, L.off, L.misc, ts, tw, th,
-                                              L.tilekey, idx2pnt);
+    bin_fill_kernel<Boxes><<<div_up(n, 256), 256, 0, s>>>(n, boxes, L.order, L.off, d_num_idx,
+                                                          capacity, L.misc, ts, tw, th,
+                                                          Q.tilekey, idx2pnt);
   }
+  const uint32_t* tsorted = Q.tilekey;
   if (capacity > 0 && nt > 1) {
     bool in_alt = false;
     uint32_t *kalt = nullptr, *valt = nullptr;
-    int rc = radix_sort_pairs<uint32_t>(s, capacity, L.misc, L.tilekey, idx2pnt,
-                                        bit_width(nt - 1), L.tsort_ws, L.tsort_bytes, &in_alt,
+    int rc = radix_sort_pairs<uint32_t>(s, capacity, L.misc, Q.tilekey, idx2pnt,
+                                        bit_width(nt - 1), Q.tsort_ws, Q.tsort_bytes, &in_alt,
                                         &kalt, &valt);
     if (rc) return rc;
-    const uint32_t* tsorted = L.tilekey;
     if (in_alt) {
       copy_u32_kernel<<<1024, 256, 0, s>>>(L.misc, capacity, valt, idx2pnt);
       tsorted = kalt;
     }
-    tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 256), 256, 0, s>>>(L.misc, capacity, nt,
-                                                                           tsorted, tile2idx);
-  } else {
-    tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 256), 256, 0, s>>>(L.misc, capacity, nt,
-                                                                           L.tilekey, tile2idx);
   }
+  tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 256), 256, 0, s>>>(L.misc, capacity, nt,
+                                                                         tsorted, tile2idx);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
+}
+
+// All phases with one workspace [points | pairs].
+template <typename Boxes>
+int bin_and_sort(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
+                 uint32_t tw, uint32_t th, uint32_t* tile2idx, uint32_t* idx2pnt,
+                 uint32_t capacity, void* ws, size_t ws_bytes, uint32_t* d_num_idx) {
+  const PointLayout L = point_layout(ws, n);
+  const PairLayout Q = pair_layout(static_cast<char*>(ws) + L.total, capacity);
+  if (ws_bytes < L.total + Q.total)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "binning: workspace too small");
+  int rc = bin_count(s, n, boxes, pixxydepth, ts, tw, th, L, d_num_idx);
+  if (!rc) rc = bin_order(s, n, L);
+  if (!rc) rc = bin_fill(s, n, boxes, ts, tw, th, L, d_num_idx, tile2idx, idx2pnt, capacity, Q);
+  return rc;
 }
 
 int check_tiles(uint32_t ts) {
@@ -222,33 +266,42 @@ int check_tiles(uint32_t ts) {
   return DSPLAT_OK;
 }
 
+// Library-allocated idx2pnt: K is read back (splat_gauss.rs:137-147) right
+// after the count, while the depth order is already queued behind it.
 template <typename Boxes>
 int bin_alloc(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
               uint32_t tw, uint32_t th, uint32_t* tile2idx, uint32_t** idx2pnt, uint32_t* num_idx) {
-  // K first, read back with a blocking copy as splat_gauss.rs:137-147 does
-  const size_t off_scan = align256(size_t(n) * 4);
-  const size_t off_k = off_scan + align256(scan_workspace_bytes(n));
+  const size_t pbytes = point_layout(nullptr, n).total;
   char* ws0 = nullptr;
-  DSPLAT_HIP(hipMallocAsync(reinterpret_cast<void**>(&ws0), off_k + 256, s));
-  uint32_t* dk = reinterpret_cast<uint32_t*>(ws0 + off_k);
-  int rc = count_pairs(s, n, boxes, ts, tw, th, reinterpret_cast<uint32_t*>(ws0),
-                       reinterpret_cast<uint32_t*>(ws0 + off_scan), dk);
+  DSPLAT_HIP(hipMallocAsync(reinterpret_cast<void**>(&ws0), pbytes + 256, s));
+  const PointLayout L = point_layout(ws0, n);
+  uint32_t* dk = reinterpret_cast<uint32_t*>(ws0 + pbytes);
   uint32_t k = 0;
+  hipEvent_t ev = nullptr;
+  int rc = bin_count(s, n, boxes, pixxydepth, ts, tw, th, L, dk);
   if (!rc) {
     hipError_t e = hipMemcpyAsync(&k, dk, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev, s);
     if (e != hipSuccess) rc = dsplat_set_hip_error(e, "binning: read back K");
   }
+  if (!rc) rc = bin_order(s, n, L);
+  if (!rc) {
+    const hipError_t e = hipEventSynchronize(ev);
+    if (e != hipSuccess) rc = dsplat_set_hip_error(e, "binning: read back K");
+  }
+  if (ev) (void)hipEventDestroy(ev);
   uint32_t* out = nullptr;
-  void* ws = nullptr;
-  const size_t wsb = bin_layout(nullptr, n, k).total;
+  void* ws1 = nullptr;
+  const size_t qbytes = pair_layout(nullptr, k).total;
   if (!rc) {
     hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&out), size_t(k ? k : 1) * 4, s);
-    if (e == hipSuccess) e = hipMallocAsync(&ws, wsb, s);
+    if (e == hipSuccess) e = hipMallocAsync(&ws1, qbytes ? qbytes : 256, s);
     if (e != hipSuccess) rc = dsplat_set_hip_error(e, "binning: allocate idx2pnt");
   }
-  if (!rc) rc = bin_and_sort(s, n, boxes, pixxydepth, ts, tw, th, tile2idx, out, k, ws, wsb, dk);
-  if (ws) (void)hipFreeAsync(ws, s);
+  if (!rc)
+    rc = bin_fill(s, n, boxes, ts, tw, th, L, dk, tile2idx, out, k, pair_layout(ws1, k));
+  if (ws1) (void)hipFreeAsync(ws1, s);
   (void)hipFreeAsync(ws0, s);
   if (rc) {
     if (out) (void)hipFreeAsync(out, s);
@@ -266,7 +319,15 @@ using namespace dsplat;
 extern "C" size_t dsplat_tile2idx_idx2pnt_workspace_size(uint32_t num_pnt, uint32_t num_tile,
                                                          uint32_t capacity) {
   (void)num_tile;
-  return bin_layout(nullptr, num_pnt, capacity).total;
+  return point_layout(nullptr, num_pnt).total + pair_layout(nullptr, capacity).total;
+}
+
+extern "C" size_t dsplat_bin_points_workspace_size(uint32_t num_pnt) {
+  return point_layout(nullptr, num_pnt).total;
+}
+
+extern "C" size_t dsplat_bin_pairs_workspace_size(uint32_t capacity) {
+  return pair_layout(nullptr, capacity).total;
 }
 
 extern "C" int dsplat_gauss_count_pairs(void* stream, uint32_t img_w, uint32_t img_h,
@@ -275,18 +336,69 @@ extern "C" int dsplat_gauss_count_pairs(void* stream, uint32_t img_w, uint32_t i
   if (int rc = check_tiles(tile_size)) return rc;
   if (!d_num_idx || (num_pnt && !pnt2pixaabb))
     return dsplat_set_error(DSPLAT_ERR_ARG, "count_pairs: null buffer");
+  if (reinterpret_cast<uintptr_t>(pnt2pixaabb) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "count_pairs: aabb must be 16-byte aligned");
   hipStream_t s = as_stream(stream);
   uint32_t tw, th;
   tile_grid(img_w, img_h, tile_size, &tw, &th);
   void* ws = nullptr;
-  const size_t b = align256(size_t(num_pnt) * 4) + scan_workspace_bytes(num_pnt);
+  const size_t b = point_layout(nullptr, num_pnt).total;
   DSPLAT_HIP(hipMallocAsync(&ws, b, s));
-  char* p = static_cast<char*>(ws);
-  int rc = count_pairs(s, num_pnt, GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)},
-                       tile_size, tw, th, reinterpret_cast<uint32_t*>(p),
-                       reinterpret_cast<uint32_t*>(p + align256(size_t(num_pnt) * 4)), d_num_idx);
+  const PointLayout L = point_layout(ws, num_pnt);
+  const int rc = bin_count(s, num_pnt, GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)},
+                           nullptr, tile_size, tw, th, L, d_num_idx);
   (void)hipFreeAsync(ws, s);
   return rc;
+}
+
+static int check_points_ws(uint32_t n, const void* ws, size_t bytes) {
+  if (!ws || bytes < point_layout(nullptr, n).total)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "binning: points workspace too small");
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_gauss_bin_count(void* stream, uint32_t img_w, uint32_t img_h,
+                                      uint32_t tile_size, uint32_t num_pnt,
+                                      const float* pnt2pixxydepth, const float* pnt2pixaabb,
+                                      void* ws_points, size_t ws_points_bytes,
+                                      uint32_t* d_num_idx) {
+  if (int rc = check_tiles(tile_size)) return rc;
+  if (!d_num_idx || (num_pnt && (!pnt2pixxydepth || !pnt2pixaabb)))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "bin_count: null buffer");
+  if (reinterpret_cast<uintptr_t>(pnt2pixaabb) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "bin_count: aabb must be 16-byte aligned");
+  if (int rc = check_points_ws(num_pnt, ws_points, ws_points_bytes)) return rc;
+  uint32_t tw, th;
+  tile_grid(img_w, img_h, tile_size, &tw, &th);
+  return bin_count(as_stream(stream), num_pnt,
+                   GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)}, pnt2pixxydepth,
+                   tile_size, tw, th, point_layout(ws_points, num_pnt), d_num_idx);
+}
+
+extern "C" int dsplat_bin_order(void* stream, uint32_t num_pnt, void* ws_points,
+                                size_t ws_points_bytes) {
+  if (int rc = check_points_ws(num_pnt, ws_points, ws_points_bytes)) return rc;
+  return bin_order(as_stream(stream), num_pnt, point_layout(ws_points, num_pnt));
+}
+
+extern "C" int dsplat_gauss_bin_fill(void* stream, uint32_t img_w, uint32_t img_h,
+                                     uint32_t tile_size, uint32_t num_pnt,
+                                     const float* pnt2pixaabb, void* ws_points,
+                                     size_t ws_points_bytes, const uint32_t* d_num_idx,
+                                     uint32_t* tile2idx, uint32_t* idx2pnt, uint32_t capacity,
+                                     void* ws_pairs, size_t ws_pairs_bytes) {
+  if (int rc = check_tiles(tile_size)) return rc;
+  if (!tile2idx || !d_num_idx || (num_pnt && !pnt2pixaabb) || (capacity && !idx2pnt))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "bin_fill: null buffer");
+  if (int rc = check_points_ws(num_pnt, ws_points, ws_points_bytes)) return rc;
+  if (!ws_pairs || ws_pairs_bytes < pair_layout(nullptr, capacity).total)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "bin_fill: pairs workspace too small");
+  uint32_t tw, th;
+  tile_grid(img_w, img_h, tile_size, &tw, &th);
+  return bin_fill(as_stream(stream), num_pnt,
+                  GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)}, tile_size, tw, th,
+                  point_layout(ws_points, num_pnt), d_num_idx, tile2idx, idx2pnt, capacity,
+                  pair_layout(ws_pairs, capacity));
 }
 
 extern "C" int dsplat_gauss_tile2idx_idx2pnt_ws(void* stream, uint32_t img_w, uint32_t img_h,
@@ -328,6 +440,39 @@ extern "C" int dsplat_gauss_tile2idx_idx2pnt(void* stream, uint32_t img_w, uint3
 }
 
 // ------------------------------------------------------------ circle path ---
+
+extern "C" int dsplat_circle_bin_count(void* stream, uint32_t tile_w, uint32_t tile_h,
+                                       uint32_t tile_size, uint32_t num_pnt,
+                                       const float* pnt2pixxydepth, const float* pnt2pixrad,
+                                       void* ws_points, size_t ws_points_bytes,
+                                       uint32_t* d_num_idx) {
+  if (int rc = check_tiles(tile_size)) return rc;
+  if (!d_num_idx || (num_pnt && (!pnt2pixxydepth || !pnt2pixrad)))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "circle bin_count: null buffer");
+  if (int rc = check_points_ws(num_pnt, ws_points, ws_points_bytes)) return rc;
+  return bin_count(as_stream(stream), num_pnt, CircleBoxes{pnt2pixxydepth, pnt2pixrad},
+                   pnt2pixxydepth, tile_size, tile_w, tile_h, point_layout(ws_points, num_pnt),
+                   d_num_idx);
+}
+
+extern "C" int dsplat_circle_bin_fill(void* stream, uint32_t tile_w, uint32_t tile_h,
+                                      uint32_t tile_size, uint32_t num_pnt,
+                                      const float* pnt2pixxydepth, const float* pnt2pixrad,
+                                      void* ws_points, size_t ws_points_bytes,
+                                      const uint32_t* d_num_idx, uint32_t* tile2idx,
+                                      uint32_t* idx2pnt, uint32_t capacity, void* ws_pairs,
+                                      size_t ws_pairs_bytes) {
+  if (int rc = check_tiles(tile_size)) return rc;
+  if (!tile2idx || !d_num_idx || (num_pnt && (!pnt2pixxydepth || !pnt2pixrad)) ||
+      (capacity && !idx2pnt))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "circle bin_fill: null buffer");
+  if (int rc = check_points_ws(num_pnt, ws_points, ws_points_bytes)) return rc;
+  if (!ws_pairs || ws_pairs_bytes < pair_layout(nullptr, capacity).total)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "circle bin_fill: pairs workspace too small");
+  return bin_fill(as_stream(stream), num_pnt, CircleBoxes{pnt2pixxydepth, pnt2pixrad},
+                  tile_size, tile_w, tile_h, point_layout(ws_points, num_pnt), d_num_idx,
+                  tile2idx, idx2pnt, capacity, pair_layout(ws_pairs, capacity));
+}
 
 extern "C" int dsplat_circle_tile2idx_idx2pnt_ws(void* stream, uint32_t tile_w, uint32_t tile_h,
                                                  uint32_t tile_size, uint32_t num_pnt,

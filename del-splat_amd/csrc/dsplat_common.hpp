@@ -80,15 +80,30 @@ __device__ inline TileRect tile_rect(float x0, float y0, float x1, float y1, uin
 
 // Per-wave LDS slice of wave_expand_pairs.
 struct ExpandLds {
+  uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there
   uint32_t off[64], x0[64], y0[64], h[64], tbase[64], tw[64], val[64];
 };
+
+// Inclusive prefix max over the 64 lanes of a wave (row shifts by 1, 2, 4,
+// 8, then row broadcasts 15 and 31; lanes shifted in from outside read 0).
+__device__ inline uint32_t wave_incl_max_u32(uint32_t v) {
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xF, 0xF, true)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xF, 0xF, true)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xF, 0xF, true)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xF, 0xF, true)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xA, 0xF, false)));
+  v = max(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xC, 0xF, false)));
+  return v;
+}
 
 // Writes the (tile, value) pairs of the calling wave's 64 consecutive items:
 // item l owns output slots [off, off + count) and covers the tile rectangle
 // r (ix-outer, iy-inner, tile id = tbase + iy * tw + ix). Items are valid on
-// a prefix of the lanes with non-decreasing offsets. Every lane writes
-// consecutive slots of the wave's range, so the stores are coalesced; the
-// item of a slot is found by binary search over the wave's offsets.
+// a prefix of the lanes with non-decreasing offsets. The wave walks its slot
+// range in chunks of 64 (every lane one slot, so the stores are coalesced);
+// the item of a slot is the last one starting at or before it: each item
+// with a non-empty range marks its first slot, and a prefix max over the
+// chunk (carried across chunks) spreads the marks.
 __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, uint32_t tbase,
                                          uint32_t tw, uint32_t val, ExpandLds& L,
                                          uint32_t* __restrict__ tile_out,
@@ -98,30 +113,41 @@ __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, u
   if (vmask == 0) return;
   const int nvalid = __popcll(vmask);
   const uint32_t cnt = valid ? uint32_t(r.count()) : 0u;
-  L.off[lane] = valid ? off : 0xffffffffu;
+  L.off[lane] = off;
   L.x0[lane] = uint32_t(r.x0);
   L.y0[lane] = uint32_t(r.y0);
   L.h[lane] = uint32_t(r.y1 - r.y0);
   L.tbase[lane] = tbase;
   L.tw[lane] = tw;
   L.val[lane] = val;
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const uint32_t base = __builtin_amdgcn_readfirstlane(off);
   const uint32_t end = __builtin_amdgcn_readlane(off + cnt, nvalid - 1);
-  for (uint32_t e = base + lane; e < end; e += 64) {
-    int lo = 0, hi = nvalid - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (L.off[mid] <= e) lo = mid;
-      else hi = mid - 1;
+  uint32_t carry = 0;  // 1 + item owning the slot before the chunk
+  for (uint32_t c0 = base; c0 < end; c0 += 64) {
+    L.own[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (cnt > 0 && off >= c0 && off - c0 < 64u) L.own[off - c0] = uint32_t(lane) + 1u;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t own = wave_incl_max_u32(max(L.own[lane], lane == 0 ? carry : 0u));
+    carry = __builtin_amdgcn_readlane(own, 63);
+    const uint32_t e = c0 + uint32_t(lane);
+    if (e < end) {
+      const uint32_t lo = own - 1u;
+      const uint32_t c = e - L.off[lo];
+      const uint32_t h = L.h[lo];
+      // c / h via the float reciprocal, then corrected to the exact
+      // quotient (off by at most one for c < 2^22; the loops make it exact)
+      uint32_t qx = uint32_t(float(c) * __builtin_amdgcn_rcpf(float(h)));
+      while (qx * h > c) --qx;
+      while ((qx + 1u) * h <= c) ++qx;
+      const uint32_t ix = L.x0[lo] + qx, iy = L.y0[lo] + (c - qx * h);
+      tile_out[e] = L.tbase[lo] + iy * L.tw[lo] + ix;
+      val_out[e] = L.val[lo];
     }
-    const uint32_t c = e - L.off[lo];
-    const uint32_t h = L.h[lo];
-    const uint32_t ix = L.x0[lo] + c / h, iy = L.y0[lo] + c % h;
-    tile_out[e] = L.tbase[lo] + iy * L.tw[lo] + ix;
-    val_out[e] = L.val[lo];
   }
   __builtin_amdgcn_wave_barrier();
 }

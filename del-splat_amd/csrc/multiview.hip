@@ -85,13 +85,6 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
 }
 
 // ------------------------------------------------------------- binning ---
-__global__ void __launch_bounds__(256)
-mv_gather_kernel(uint32_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ src,
-                 uint32_t* __restrict__ dst) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) dst[j] = src[order[j]];
-}
-
 __global__ void mv_clamp_kernel(uint32_t* __restrict__ misc, uint32_t capacity) {
   const uint32_t k = misc[0];
   misc[1] = k <= capacity ? k : 0u;
@@ -517,12 +510,13 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
 #undef DSPLAT_MVP
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
-  // values: list items (iota), generated by the first pass
+  // values: list items (iota), generated by the first pass; the last pass
+  // also gathers the tile counts into depth order
   int rc = radix_sort_pairs<uint32_t>(s, items, nullptr, P->dkey, P->order, 31, P->dsort_ws,
-                                      P->dsort_bytes, &in_alt, &kalt, &valt, true);
+                                      P->dsort_bytes, &in_alt, &kalt, &valt, true, P->cnt,
+                                      P->off);
   if (rc) return rc;
   const uint32_t* order = in_alt ? valt : P->order;
-  mv_gather_kernel<<<div_up(items, 256), 256, 0, s>>>(items, order, P->cnt, P->off);
   rc = exclusive_scan(s, items, nullptr, P->off, P->off, P->scan_ws, P->misc);
   if (rc) return rc;
   mv_clamp_kernel<<<1, 1, 0, s>>>(P->misc, P->capacity);

@@ -233,14 +233,16 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
   if (threadIdx.x == 0) tot[d] = carry;
 }
 
-// iota: the values are the element indices (vin is not read).
+// iota: the values are the element indices (vin is not read). gdst (may be
+// null): also writes gdst[pos] = gsrc[value].
 template <typename KeyT>
 __global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
                      const uint32_t* __restrict__ vin, bool iota, KeyT* __restrict__ kout,
                      uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
                      uint32_t nb, const uint32_t* __restrict__ off,
-                     const uint32_t* __restrict__ tot, uint32_t nbins) {
+                     const uint32_t* __restrict__ tot, uint32_t nbins,
+                     const uint32_t* __restrict__ gsrc, uint32_t* __restrict__ gdst) {
   static_assert(kSortBlock == 256, "one thread per digit");
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dpre[256];
@@ -312,8 +314,10 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     const KeyT k = skey[i];
     const uint32_t d = uint32_t(k >> shift) & mask;
     const uint32_t pos = dbase[d] + (i - dpre[d]);
+    const uint32_t v = sval[i];
     kout[pos] = k;
-    vout[pos] = sval[i];
+    vout[pos] = v;
+    if (gdst) gdst[pos] = gsrc[v];
   }
 }
 
@@ -326,7 +330,7 @@ template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out,
-                     bool iota_vals) {
+                     bool iota_vals, const uint32_t* gather_src, uint32_t* gather_dst) {
   *result_in_alt = false;
   if (ws_bytes < sort_workspace_bytes<KeyT>(n_cap))
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
@@ -347,9 +351,10 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
     radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb,
                                                          L.cnt);
     radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, L.cnt, L.tot);
+    const bool last = p + 1 == P;
     radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(
         n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt, L.tot,
-        1u << dbits);
+        1u << dbits, gather_src, last ? gather_dst : nullptr);
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;
@@ -363,10 +368,10 @@ template size_t sort_workspace_bytes<uint32_t>(uint32_t);
 template size_t sort_workspace_bytes<uint64_t>(uint32_t);
 template int radix_sort_pairs<uint32_t>(hipStream_t, uint32_t, const uint32_t*, uint32_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint32_t**,
-                                        uint32_t**, bool);
+                                        uint32_t**, bool, const uint32_t*, uint32_t*);
 template int radix_sort_pairs<uint64_t>(hipStream_t, uint32_t, const uint32_t*, uint64_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint64_t**,
-                                        uint32_t**, bool);
+                                        uint32_t**, bool, const uint32_t*, uint32_t*);
 
 // ------------------------------------------------------------------ fill ----
 __global__ void fill_f32_kernel(uint32_t n, float* __restrict__ buf, float v) {

@@ -23,12 +23,14 @@ size_t sort_workspace_bytes(uint32_t n);
 // *result_in_alt is false, else in the workspace's alternate buffers
 // (*keys_alt, *vals_alt). iota_vals: the values are the indices 0..n-1 and
 // vals is not read (it still receives the result when *result_in_alt is
-// false).
+// false). gather_dst (may be null): the last pass also writes
+// gather_dst[i] = gather_src[sorted value i].
 template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt,
-                     bool iota_vals = false);
+                     bool iota_vals = false, const uint32_t* gather_src = nullptr,
+                     uint32_t* gather_dst = nullptr);
 
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 

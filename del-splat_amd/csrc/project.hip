@@ -40,9 +40,9 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
   }
 }
 
-// Adjoint: adds the chain rule from (pixel centre, conic) into the mean,
-// scale and quaternion gradients.
-template <bool kLogScale>
+// Adjoint: the chain rule from (pixel centre, conic) into the mean, scale
+// and quaternion gradients, added to them or (kOverwrite) stored.
+template <bool kLogScale, bool kOverwrite>
 __global__ void __launch_bounds__(256)
 gauss_project_bwd_kernel(uint32_t n, const float* __restrict__ xyz,
                          const float* __restrict__ scale, const float4* __restrict__ quat,
@@ -62,15 +62,17 @@ gauss_project_bwd_kernel(uint32_t n, const float* __restrict__ xyz,
   const float gcon[3] = {d_conic[3 * i], d_conic[3 * i + 1], d_conic[3 * i + 2]};
   float dx[3], dsc[3], dq[4];
   if (!project_bwd_one(m, img_w, img_h, x3, quat[i], sg, kLogScale, d_pixxy[i], gcon, dx, dsc,
-                       dq))
-    return;
-  d_xyz[3 * i + 0] += dx[0];
-  d_xyz[3 * i + 1] += dx[1];
-  d_xyz[3 * i + 2] += dx[2];
-  d_scale[3 * i + 0] += dsc[0];
-  d_scale[3 * i + 1] += dsc[1];
-  d_scale[3 * i + 2] += dsc[2];
-  float4 q = d_quat[i];
+                       dq)) {
+    if (!kOverwrite) return;
+    dx[0] = dx[1] = dx[2] = dsc[0] = dsc[1] = dsc[2] = 0.f;
+    dq[0] = dq[1] = dq[2] = dq[3] = 0.f;
+  }
+  float4 q = kOverwrite ? make_float4(0.f, 0.f, 0.f, 0.f) : d_quat[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    d_xyz[3 * i + c] = kOverwrite ? dx[c] : d_xyz[3 * i + c] + dx[c];
+    d_scale[3 * i + c] = kOverwrite ? dsc[c] : d_scale[3 * i + c] + dsc[c];
+  }
   q.x += dq[0];
   q.y += dq[1];
   q.z += dq[2];
@@ -161,10 +163,16 @@ extern "C" int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const fl
   const float4* q4 = reinterpret_cast<const float4*>(pnt2quat);
   const float2* gp = reinterpret_cast<const float2*>(d_pnt2pixxy);
   float4* dq = reinterpret_cast<float4*>(d_pnt2quat);
-  if (flags & DSPLAT_SCALE_IS_LOG)
-    gauss_project_bwd_kernel<true><<<nb, 256, 0, s>>>(num_pnt, pnt2xyz, pnt2scale, q4, transform_world2ndc, img_w, img_h, gp, d_pnt2conic, d_pnt2xyz, d_pnt2scale, dq);
-  else
-    gauss_project_bwd_kernel<false><<<nb, 256, 0, s>>>(num_pnt, pnt2xyz, pnt2scale, q4, transform_world2ndc, img_w, img_h, gp, d_pnt2conic, d_pnt2xyz, d_pnt2scale, dq);
+#define DSPLAT_PBWD(L_, O_)                                                                \
+  gauss_project_bwd_kernel<L_, O_><<<nb, 256, 0, s>>>(num_pnt, pnt2xyz, pnt2scale, q4,   \
+                                                      transform_world2ndc, img_w, img_h, gp, \
+                                                      d_pnt2conic, d_pnt2xyz, d_pnt2scale, dq)
+  const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, overwrite = flags & DSPLAT_GRAD_OVERWRITE;
+  if (log_scale && overwrite) DSPLAT_PBWD(true, true);
+  else if (log_scale) DSPLAT_PBWD(true, false);
+  else if (overwrite) DSPLAT_PBWD(false, true);
+  else DSPLAT_PBWD(false, false);
+#undef DSPLAT_PBWD
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }

@@ -8,7 +8,7 @@ import torch
 
 from .. import _CapsuleAsDLPack, _native, util_torch
 from .._native import call
-from .._torch_util import ptr, require_gpu, stream_ptr, tile_shape
+from .._torch_util import bin_phases, ptr, require_gpu, stream_ptr, tile_shape
 
 
 def from_project_spheres(pnt2xyz: torch.Tensor, radius: float, transform_world2ndc: torch.Tensor,
@@ -43,23 +43,19 @@ def tile2idx_idx2pnt(tile_shape_, tile_size, pnt2pixxydepth, pnt2pixrad):
     dev = pnt2pixxydepth.device
     n = pnt2pixxydepth.shape[0]
     tw, th = tile_shape_
-    s = stream_ptr(dev)
     pxd = pnt2pixxydepth.contiguous()
     rad = pnt2pixrad.contiguous()
     # boxes [x-r, y-r, x+r, y+r] counted on the device, K read back once
-    kbuf = torch.zeros(4, device=dev, dtype=torch.int32)
-    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, 0)
-    ws = torch.empty(ws_bytes, device=dev, dtype=torch.uint8)
-    t2i = torch.empty(tw * th + 1, device=dev, dtype=torch.int32)
-    call("dsplat_circle_tile2idx_idx2pnt_ws", s, tw, th, tile_size, n, ptr(pxd), ptr(rad),
-         ptr(t2i), 0, 0, ptr(ws), ws_bytes, ptr(kbuf))
-    k = int(kbuf[0].item())
-    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, k)
-    ws = torch.empty(ws_bytes, device=dev, dtype=torch.uint8)
-    i2p = torch.empty(max(k, 1), device=dev, dtype=torch.int32)
-    call("dsplat_circle_tile2idx_idx2pnt_ws", s, tw, th, tile_size, n, ptr(pxd), ptr(rad),
-         ptr(t2i), ptr(i2p), k, ptr(ws), ws_bytes, ptr(kbuf))
-    return t2i, i2p[:k]
+
+    def count(s, wsp, wsp_bytes, kbuf):
+        call("dsplat_circle_bin_count", s, tw, th, tile_size, n, ptr(pxd), ptr(rad), ptr(wsp),
+             wsp_bytes, ptr(kbuf))
+
+    def fill(s, wsp, wsp_bytes, kbuf, t2i, i2p, k, wsq, wsq_bytes):
+        call("dsplat_circle_bin_fill", s, tw, th, tile_size, n, ptr(pxd), ptr(rad), ptr(wsp),
+             wsp_bytes, ptr(kbuf), ptr(t2i), ptr(i2p), k, ptr(wsq), wsq_bytes)
+
+    return bin_phases(dev, n, tw * th, count, fill)
 
 
 def splat(img_shape, pnt2pixxydepth, pnt2pixrad, pnt2rgb, tile_size, tile2idx, idx2pnt):

@@ -11,7 +11,7 @@ import torch
 
 from .. import _native
 from .._native import call
-from .._torch_util import (check_f32, ptr, require_gpu, stream_ptr, tile_shape,
+from .._torch_util import (bin_phases, check_f32, ptr, require_gpu, stream_ptr, tile_shape,
                            world2ndc_col_major)
 
 DEFAULT_TILE = 16
@@ -39,19 +39,25 @@ def pnt2splat3_to_pnt2splat2(pnt2xyz, pnt2scale, pnt2quat, transform_world2ndc, 
 def tile2idx_idx2pnt(img_shape, tile_size, pnt2pixxydepth, pnt2pixaabb):
     """Binning + stable sort (splat_gauss.rs:96-195): returns (tile2idx
     u32[T+1], idx2pnt u32[K]) as int32-typed tensors holding u32 values.
-    Reads K back with one blocking copy, as the reference does."""
+    Reads K back once, as the reference does; the depth order runs on the
+    GPU meanwhile."""
     require_gpu(pnt2pixaabb, "pnt2pixaabb")
     dev = pnt2pixaabb.device
     n = pnt2pixaabb.shape[0]
+    w, h = img_shape
     tw, th = tile_shape(img_shape, tile_size)
-    s = stream_ptr(dev)
-    kbuf = torch.zeros(4, device=dev, dtype=torch.int32)
     pxd = pnt2pixxydepth.contiguous()
     bb = pnt2pixaabb.contiguous()
-    call("dsplat_gauss_count_pairs", s, img_shape[0], img_shape[1], tile_size, n, ptr(bb),
-         ptr(kbuf))
-    k = int(kbuf[0].item())
-    return _bin_with_capacity(img_shape, tile_size, n, pxd, bb, k, kbuf)
+
+    def count(s, wsp, wsp_bytes, kbuf):
+        call("dsplat_gauss_bin_count", s, w, h, tile_size, n, ptr(pxd), ptr(bb), ptr(wsp),
+             wsp_bytes, ptr(kbuf))
+
+    def fill(s, wsp, wsp_bytes, kbuf, t2i, i2p, k, wsq, wsq_bytes):
+        call("dsplat_gauss_bin_fill", s, w, h, tile_size, n, ptr(bb), ptr(wsp), wsp_bytes,
+             ptr(kbuf), ptr(t2i), ptr(i2p), k, ptr(wsq), wsq_bytes)
+
+    return bin_phases(dev, n, tw * th, count, fill)
 
 
 def _bin_with_capacity(img_shape, tile_size, n, pxd, bb, cap, kbuf):
@@ -120,10 +126,12 @@ def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast,
     dev = splat2.device
     n = splat2.shape[0]
     w, h = img_shape
-    d_pixxy = torch.zeros((n, 2), device=dev, dtype=torch.float32)
-    d_conic = torch.zeros((n, 3), device=dev, dtype=torch.float32)
-    d_op = torch.zeros(n, device=dev, dtype=torch.float32)
-    d_rgb = torch.zeros((n, 3), device=dev, dtype=torch.float32)
+    # the adjoint accumulates: one zero fill for all four
+    acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
+    d_pixxy = acc[:2 * n].view(n, 2)
+    d_conic = acc[2 * n:5 * n].view(n, 3)
+    d_op = acc[5 * n:6 * n]
+    d_rgb = acc[6 * n:].view(n, 3)
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
     da = None if d_alpha is None else d_alpha.contiguous()
     call("dsplat_gauss_rasterize_bwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
@@ -136,12 +144,13 @@ def project_bwd(xyz, scale, quat, m16, img_shape, d_pixxy, d_conic, log_scale=Tr
     """Returns (d_xyz [N,3], d_scale [N,3], d_quat [N,4])."""
     dev = xyz.device
     n = xyz.shape[0]
-    d_xyz = torch.zeros((n, 3), device=dev, dtype=torch.float32)
-    d_scale = torch.zeros((n, 3), device=dev, dtype=torch.float32)
-    d_quat = torch.zeros((n, 4), device=dev, dtype=torch.float32)
+    d_xyz = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    d_scale = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    d_quat = torch.empty((n, 4), device=dev, dtype=torch.float32)
+    flags = _native.DSPLAT_GRAD_OVERWRITE | (_native.DSPLAT_SCALE_IS_LOG if log_scale else 0)
     call("dsplat_gauss_project_bwd", stream_ptr(dev), n, ptr(xyz), ptr(scale), ptr(quat),
-         ptr(m16), img_shape[0], img_shape[1], _native.DSPLAT_SCALE_IS_LOG if log_scale else 0,
-         ptr(d_pixxy), ptr(d_conic), ptr(d_xyz), ptr(d_scale), ptr(d_quat))
+         ptr(m16), img_shape[0], img_shape[1], flags, ptr(d_pixxy), ptr(d_conic), ptr(d_xyz),
+         ptr(d_scale), ptr(d_quat))
     return d_xyz, d_scale, d_quat
 
 

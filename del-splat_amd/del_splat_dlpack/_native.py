@@ -20,6 +20,7 @@ _u64 = ctypes.c_uint64
 
 DSPLAT_SCALE_IS_LOG = 1
 DSPLAT_OPACITY_IS_LOGIT = 2
+DSPLAT_GRAD_OVERWRITE = 4
 
 
 class ViewDesc(ctypes.Structure):
@@ -36,6 +37,13 @@ _SIGS = {
     "dsplat_tile2idx_idx2pnt_workspace_size": (_sz, [_u32, _u32, _u32]),
     "dsplat_gauss_tile2idx_idx2pnt_ws": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _sz, _vp]),
     "dsplat_gauss_tile2idx_idx2pnt": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_u32)]),
+    "dsplat_bin_points_workspace_size": (_sz, [_u32]),
+    "dsplat_bin_pairs_workspace_size": (_sz, [_u32]),
+    "dsplat_gauss_bin_count": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "dsplat_bin_order": (_int, [_vp, _u32, _vp, _sz]),
+    "dsplat_gauss_bin_fill": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _sz, _vp, _vp, _vp, _u32, _vp, _sz]),
+    "dsplat_circle_bin_count": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _sz, _vp]),
+    "dsplat_circle_bin_fill": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _sz, _vp, _vp, _vp, _u32, _vp, _sz]),
     "dsplat_free": (_int, [_vp, _vp]),
     "dsplat_gauss_rasterize_pnt2splat2": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
     "dsplat_gauss_project_fwd": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),

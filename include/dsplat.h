@@ -88,6 +88,30 @@ int dsplat_gauss_tile2idx_idx2pnt_ws(void* stream, uint32_t img_w, uint32_t img_
                                      void* workspace, size_t workspace_bytes,
                                      uint32_t* d_num_idx);
 
+/* The same binning split into phases, for callers that allocate idx2pnt
+ * once K is known (the reference reads K back between counting and filling,
+ * splat_gauss.rs:137-147), so the depth order can run while K travels to the
+ * host. On one stream, in this order:
+ *   dsplat_gauss_bin_count  K into d_num_idx (device u32)
+ *   dsplat_bin_order        stable depth order of the Gaussians
+ *   dsplat_gauss_bin_fill   tile2idx / idx2pnt, as the _ws variant with this
+ *                           capacity (nothing binned when K > capacity)
+ * ws_points (dsplat_bin_points_workspace_size(num_pnt) bytes) carries the
+ * state from phase to phase; ws_pairs holds
+ * dsplat_bin_pairs_workspace_size(capacity) bytes. */
+size_t dsplat_bin_points_workspace_size(uint32_t num_pnt);
+size_t dsplat_bin_pairs_workspace_size(uint32_t capacity);
+int dsplat_gauss_bin_count(void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size,
+                           uint32_t num_pnt, const float* pnt2pixxydepth,
+                           const float* pnt2pixaabb, void* ws_points, size_t ws_points_bytes,
+                           uint32_t* d_num_idx);
+int dsplat_bin_order(void* stream, uint32_t num_pnt, void* ws_points, size_t ws_points_bytes);
+int dsplat_gauss_bin_fill(void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size,
+                          uint32_t num_pnt, const float* pnt2pixaabb, void* ws_points,
+                          size_t ws_points_bytes, const uint32_t* d_num_idx, uint32_t* tile2idx,
+                          uint32_t* idx2pnt, uint32_t capacity, void* ws_pairs,
+                          size_t ws_pairs_bytes);
+
 /* SYNC. Same result as the _ws variant with the reference's ownership: the
  * library allocates idx2pnt (exactly K entries, returned in *idx2pnt and the
  * host *num_idx; release it with dsplat_free) after a blocking read of K, as
@@ -155,7 +179,10 @@ int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                float* d_pnt2rgb);
 
 /* Adjoint of the projection. ADDS into d_pnt2xyz [N,3], d_pnt2scale [N,3]
- * (w.r.t. sigma, or log sigma under DSPLAT_SCALE_IS_LOG), d_pnt2quat [N,4]. */
+ * (w.r.t. sigma, or log sigma under DSPLAT_SCALE_IS_LOG), d_pnt2quat [N,4];
+ * with DSPLAT_GRAD_OVERWRITE in flags it stores them instead (no zeroing
+ * needed beforehand). */
+#define DSPLAT_GRAD_OVERWRITE 4u
 int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,
                              const float* pnt2scale, const float* pnt2quat,
                              const float* transform_world2ndc, uint32_t img_w, uint32_t img_h,
@@ -193,6 +220,18 @@ int dsplat_circle_tile2idx_idx2pnt(void* stream, uint32_t tile_w, uint32_t tile_
                                    uint32_t tile_size, uint32_t num_pnt,
                                    const float* pnt2pixxydepth, const float* pnt2pixrad,
                                    uint32_t* tile2idx, uint32_t** idx2pnt, uint32_t* num_idx);
+
+/* Phases of the circle binning (see dsplat_gauss_bin_count; the order phase
+ * is dsplat_bin_order). */
+int dsplat_circle_bin_count(void* stream, uint32_t tile_w, uint32_t tile_h, uint32_t tile_size,
+                            uint32_t num_pnt, const float* pnt2pixxydepth,
+                            const float* pnt2pixrad, void* ws_points, size_t ws_points_bytes,
+                            uint32_t* d_num_idx);
+int dsplat_circle_bin_fill(void* stream, uint32_t tile_w, uint32_t tile_h, uint32_t tile_size,
+                           uint32_t num_pnt, const float* pnt2pixxydepth,
+                           const float* pnt2pixrad, void* ws_points, size_t ws_points_bytes,
+                           const uint32_t* d_num_idx, uint32_t* tile2idx, uint32_t* idx2pnt,
+                           uint32_t capacity, void* ws_pairs, size_t ws_pairs_bytes);
 
 /* Tiled painter: per pixel the last list entry (ascending key order) whose
  * circle covers the pixel centre gives the colour; uncovered pixels are 0.

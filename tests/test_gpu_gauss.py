@@ -103,6 +103,29 @@ def test_binning_allocating_variant(oracle):
     np.testing.assert_array_equal(u32(buf), o_i2p)
 
 
+def test_binning_one_call_and_count_only(oracle):
+    """The one-call _ws entry at capacity K and above, and the count-only
+    entry, against the oracle's lists; empty input through the phased path."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import _native
+    shape = (200, 150)
+    s, m = small_scene(4000, 16, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, 16)
+    kbuf = torch.zeros(4, device=DEV, dtype=torch.int32)
+    _native.call("dsplat_gauss_count_pairs", torch.cuda.current_stream().cuda_stream, shape[0],
+                 shape[1], 16, 4000, aabb.data_ptr(), kbuf.data_ptr())
+    assert int(kbuf[0].item()) == len(o_i2p)
+    for cap in (len(o_i2p), len(o_i2p) + 1000):
+        t2i, i2p = SG._bin_with_capacity(shape, 16, 4000, pxd, aabb, cap, kbuf)
+        assert int(kbuf[0].item()) == len(o_i2p)
+        np.testing.assert_array_equal(u32(t2i), o_t2i)
+        np.testing.assert_array_equal(u32(i2p)[:len(o_i2p)], o_i2p)
+    e3, e4 = torch.zeros((0, 3), device=DEV), torch.zeros((0, 4), device=DEV)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, e3, e4)
+    assert i2p.numel() == 0 and int(t2i.abs().sum().item()) == 0
+
+
 def _copy_device(dst, src, nbytes):
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
@@ -307,3 +330,25 @@ def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
         err = max_rel(a.cpu().numpy(), b)
         print(f"ts={ts} threads={threads} alpha={with_alpha} {name}: max rel err {err:.2e}")
         assert err <= 2e-3, (name, err)
+
+
+def test_project_bwd_add_and_overwrite():
+    """dsplat_gauss_project_bwd adds into the gradient buffers by default and
+    stores into them with DSPLAT_GRAD_OVERWRITE: add(into ones) == 1 + store."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import _native
+    shape = (160, 120)
+    s, m = small_scene(3000, 77, shape)
+    xyz, ls, q = _t(s["xyz"]), _t(s["log_scale"]), _t(s["quat"])
+    n = xyz.shape[0]
+    rng = np.random.default_rng(5)
+    dpix = _t(rng.standard_normal((n, 2)).astype(np.float32))
+    dcon = _t(rng.standard_normal((n, 3)).astype(np.float32))
+    store = SG.project_bwd(xyz, ls, q, _t(m), shape, dpix, dcon)
+    add = [torch.ones((n, k), device=DEV) for k in (3, 3, 4)]
+    _native.call("dsplat_gauss_project_bwd", torch.cuda.current_stream().cuda_stream, n,
+                 xyz.data_ptr(), ls.data_ptr(), q.data_ptr(), _t(m).data_ptr(), shape[0],
+                 shape[1], _native.DSPLAT_SCALE_IS_LOG, dpix.data_ptr(), dcon.data_ptr(),
+                 *[a.data_ptr() for a in add])
+    for a, b in zip(add, store):
+        np.testing.assert_allclose(a.cpu().numpy(), 1.0 + b.cpu().numpy(), rtol=1e-6, atol=1e-6)
