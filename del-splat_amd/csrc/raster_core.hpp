@@ -60,22 +60,28 @@ struct ArrayLoader {
   }
 };
 
-__device__ inline float fast_exp(float x) { return __expf(x); }
+// The splat weight exp(-q/2), q = d^T C d for d = (dx, dy) and C = [[a, b],
+// [b, c]], as 2^(d^T C' d) with C' = -log2(e)/2 C formed once per list entry
+// (wave-uniform). One fixed evaluation order (contraction off; fused only
+// where fmaf is written), shared by the forward and the adjoint, so every
+// workgroup shape produces bit-identical pixels and the adjoint recomputes
+// exactly the forward's weights.
+struct ConicE2 {
+  float a, b2, c;  // scaled a, 2b, c
+};
 
-// The splat weight's pieces in one fixed evaluation order (contraction off;
-// fused only where fmaf is written), shared by the forward and the adjoint,
-// so every workgroup shape produces bit-identical pixels and the adjoint
-// recomputes exactly the forward's e.
-// q = d^T C d for d = (dx, dy), C = [[a, b], [b, c]].
-__device__ __forceinline__ float conic_q(float a, float b, float c, float dx, float dy) {
+__device__ __forceinline__ ConicE2 conic_e2(float a, float b, float c) {
 #pragma clang fp contract(off)
-  return fmaf(a * dx, dx, fmaf((b + b) * dx, dy, c * dy * dy));
+  constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
+  return {a * kS, (b + b) * kS, c * kS};
 }
-// exp(-q/2) where in, else 0 (exp(-inf) = 0: a select on the argument
-// keeps the exp unpredicated)
-__device__ __forceinline__ float splat_kernel(bool in, float q) {
+
+// exp(-q/2) where in, else 0 (2^-inf = 0: a select on the argument keeps
+// the exp unpredicated).
+__device__ __forceinline__ float splat_weight(bool in, const ConicE2& k, float dx, float dy) {
 #pragma clang fp contract(off)
-  return fast_exp(in ? -0.5f * q : -__builtin_inff());
+  const float x = fmaf(k.a * dx, dx, fmaf(k.b2 * dx, dy, k.c * dy * dy));
+  return __builtin_amdgcn_exp2f(in ? x : -__builtin_inff());
 }
 
 // Pixel -> thread mapping of a tile (forward and adjoint). The TS x TS tile
@@ -130,10 +136,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q;
   __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
   const uint32_t num = end - start;
+  // A pixel is live while !(alpha_sum >= 0.999f) (NaN stays live, as in the
+  // reference's test); pixels outside the image start at alpha_sum = 1.
   float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
   uint32_t walked[PPT];
-  bool done[PPT];
-  int live = 0;
   // pixel-centre bounds of each group's region (wave-uniform)
   float gx0[G], gy0[G], gx1[G], gy1[G];
 #pragma unroll
@@ -148,11 +154,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     const uint32_t ix = tx * TS + GM::lx(k), iy = ty * TS + GM::ly(k);
     px[k] = float(ix) + 0.5f;
     py[k] = float(iy) + 0.5f;
-    cr[k] = cg[k] = cbl[k] = asum[k] = 0.f;
+    cr[k] = cg[k] = cbl[k] = 0.f;
+    asum[k] = (ix < img_w && iy < img_h) ? 0.f : 1.f;
     tr[k] = tl[k] = 1.f;
     walked[k] = 0;
-    done[k] = !(ix < img_w && iy < img_h);
-    live += done[k] ? 0 : 1;
   }
   for (uint32_t b = 0; b < num; b += NT) {
     const uint32_t it = b + threadIdx.x;
@@ -170,7 +175,14 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     // pixel) gets e = 0, which leaves every accumulator bit-identical
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
     for (uint32_t j = 0; j < cnt; ++j) {
-      if (__ballot(live > 0) == 0) break;  // every pixel of the wave is done
+      bool live[PPT];
+      bool anylive = false;
+#pragma unroll
+      for (int k = 0; k < PPT; ++k) {
+        live[k] = !(asum[k] >= 0.999f);
+        anylive = anylive || live[k];
+      }
+      if (__ballot(anylive) == 0) break;  // every pixel of the wave is done
       const float4 q2 = s2[j];
       const float4 q3 = s3[j];
       bool hit[G];
@@ -183,6 +195,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       if (!anyg) continue;  // the box reaches none of the wave's regions
       const float4 q0 = s0[j];
       const float4 q1 = s1[j];
+      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (!hit[g]) continue;  // wave-uniform
@@ -192,7 +205,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
         for (int qq = 0; qq < Q; ++qq) {
           const int k = g * Q + qq;
           // inclusive box test (aabb2::is_include_point)
-          inm[qq] = !done[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z && py[k] <= q3.x;
+          // non-short-circuit &: plain compares and mask ands, no branches
+          inm[qq] = live[k] & (px[k] >= q2.y) & (px[k] <= q2.w) & (py[k] >= q2.z) &
+                    (py[k] <= q3.x);
           any = any || inm[qq];
         }
         if (__ballot(any) == 0) continue;  // no live pixel of the group is inside the box
@@ -201,8 +216,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           const int k = g * Q + qq;
           const bool in = inm[qq];
           const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-          const float q = conic_q(q0.z, q0.w, q1.x, dx, dy);
-          const float e = splat_kernel(in, q) * q1.y;
+          const float e = splat_weight(in, ke, dx, dy) * q1.y;
           const float eo = tr[k] * e;
           cr[k] = fmaf(q1.z, eo, cr[k]);
           cg[k] = fmaf(q1.w, eo, cg[k]);
@@ -211,13 +225,13 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           walked[k] = in ? b + j + 1 : walked[k];
           tr[k] = tr[k] * (1.f - e);
           asum[k] = asum[k] + eo;
-          const bool term = in && asum[k] >= 0.999f;
-          done[k] = done[k] || term;
-          live -= term ? 1 : 0;
         }
       }
     }
-    if (__syncthreads_and(live == 0)) break;
+    bool lane_live = false;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) lane_live = lane_live || !(asum[k] >= 0.999f);
+    if (__syncthreads_and(!lane_live)) break;
   }
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
@@ -419,6 +433,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       }
       if (!anyg) continue;  // the box reaches none of the wave's regions
       const float4 q0 = s0[j], q1 = s1[j];
+      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
@@ -431,8 +446,8 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
 #pragma unroll
         for (int qq = 0; qq < Q; ++qq) {
           const int k = g * Q + qq;
-          inm[qq] = it < nw[k] && px[k] >= q2.y && px[k] <= q2.w && py[k] >= q2.z &&
-                    py[k] <= q3.x;
+          inm[qq] = (it < nw[k]) & (px[k] >= q2.y) & (px[k] <= q2.w) & (py[k] >= q2.z) &
+                    (py[k] <= q3.x);
           any = any || inm[qq];
         }
         if (__ballot(any) == 0) continue;  // no pixel of the group takes this entry
@@ -444,8 +459,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           const int k = g * Q + qq;
           const bool in = inm[qq];
           const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-          const float q = conic_q(q0.z, q0.w, q1.x, dx, dy);
-          const float gk = splat_kernel(in, q);
+          const float gk = splat_weight(in, ke, dx, dy);
           const float e = gk * q1.y;
           const float ome = 1.f - e;
           const float inv = __builtin_amdgcn_rcpf(ome);
