@@ -295,6 +295,33 @@ __device__ inline void wave_sums_to_last(float (&v)[N]) {
   for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x143, 0xC>(v[c]);
 }
 
+// Sums of 8 values over the wave, as a reduce-scatter: three exchange
+// levels inside each 8-lane group (half-row mirror, quad swaps) leave lane r
+// with its group's sum of value r & 7 (3 + 6 + ... = 21 instructions for all
+// eight values instead of 8 full reductions), then a rotate by 8 within the
+// row and two cross-row exchanges complete the sums. Every lane returns the
+// total of value (lane & 7).
+__device__ inline float wave_sum8_scatter(const float (&v)[8], int lane) {
+  const bool b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
+  float w[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float keep = b2 ? v[c + 4] : v[c], send = b2 ? v[c] : v[c + 4];
+    w[c] = keep + dpp_mov<0x141>(send);  // row_half_mirror: lane 7 - i of the group
+  }
+  float x[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const float keep = b1 ? w[c + 2] : w[c], send = b1 ? w[c] : w[c + 2];
+    x[c] = keep + dpp_mov<0x4E>(send);  // quad_perm [2,3,0,1]
+  }
+  float y = (b0 ? x[1] : x[0]) + dpp_mov<0xB1>(b0 ? x[0] : x[1]);  // quad_perm [1,0,3,2]
+  y += dpp_mov<0x128>(y);  // row_ror:8
+  y += __shfl_xor(y, 16, 64);
+  y += __shfl_xor(y, 32, 64);
+  return y;
+}
+
 // Reduced per list entry: S_x, S_y, S_xx, S_xy, S_yy (moments of dL/dq over
 // the pixel offsets), dL/dopacity, dL/drgb.
 constexpr int kGrad = 9;
@@ -487,11 +514,12 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         }
       }
       if (!touched) continue;
-      wave_sums_to_last(v);
-      if (lane == 63) {
-#pragma unroll
-        for (int c = 0; c < kGrad; ++c) acc[c][j] = v[c];
-      }
+      const float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+      const float tot = wave_sum8_scatter(v8, lane);
+      float vb[1] = {v[8]};
+      wave_sums_to_last(vb);
+      if (lane < 8) acc[lane][j] = tot;
+      if (lane == 63) acc[8][j] = vb[0];
     }
     __syncthreads();
     if (threadIdx.x < cnt) {
