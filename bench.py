@@ -2,10 +2,12 @@
 
 Headline (one step = one frame): 1M seeded Gaussians (means uniform in a
 radius-2 ball, log-scales N(-4,0.5), random unit quaternions, opacity
-sigmoid(N(0,1)), colours U[0,1]), camera at distance 3, fov 60 deg, 1920x1080,
-16x16 tiles; loss = sum (rgb - target)^2 against a seeded U[0,1] image. A step
-runs projection, binning + stable sort, tiled raster, the loss gradient, the
-raster adjoint and the projection adjoint, inputs resident in HBM.
+sigmoid(N(0,1)), colours U[0,1]), camera at distance 3, fov 60 deg, 1920x1080;
+tile size and forward / adjoint workgroup sizes from the autotuner (or
+--tile / --threads / --threads-bwd); loss = sum (rgb - target)^2 against a
+seeded U[0,1] image. A step runs projection, binning + stable sort, tiled
+raster, the loss gradient, the raster adjoint and the projection adjoint,
+inputs resident in HBM.
 
 Batched section: the same 1M Gaussians seen by 16 cameras on a radius-3
 sphere with resolutions drawn from {640x480, 1280x720, 1920x1080, 800x800};
@@ -14,8 +16,12 @@ versus the per-view path (the single-view pipeline called once per view);
 and forward-only on 2M Gaussians in the unit ball. Tile size and waves per
 tile of the batched path come from the autotuner.
 
+Training section: 3M Gaussians, 8 uneven views drawn afresh per step, fused
+forward + L2 adjoint + Adam on all parameters.
+
 Prints one JSON line.
   python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu-baseline] [--no-batched]
+                  [--no-train]
 """
 import argparse
 import json
@@ -342,9 +348,11 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs,
-        # keyed by kernel and tile size (the traffic depends on the schedule)
+        # keyed by kernel, tile size and workgroup size (the traffic depends on
+        # the schedule)
+        th = {"raster_fwd": args.threads, "raster_bwd": args.threads_bwd}.get(dom, 0)
         try:
-            entry = json.load(open(tpath)).get(f"{dom}@tile{args.tile}")
+            entry = json.load(open(tpath)).get(f"{dom}@tile{args.tile}/{th}")
             traffic = None if entry is None else entry["bytes"]
         except Exception:
             traffic = None
