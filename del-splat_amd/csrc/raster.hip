@@ -87,6 +87,8 @@ int launch_bwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
   } else if (ts == 32) {
     if (threads == 64) DSPLAT_BWD(32, 64);
     else if (threads == 128) DSPLAT_BWD(32, 128);
+    else if (threads == 512) DSPLAT_BWD(32, 512);
+    else if (threads == 1024) DSPLAT_BWD(32, 1024);
     else DSPLAT_BWD(32, 256);
   } else {
     return dsplat_set_error(DSPLAT_ERR_ARG, "tile_size must be 8, 16 or 32");

@@ -134,7 +134,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
 #pragma clang fp contract(off)
   using GM = GroupMap<TS, NT>;
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q;
-  __shared__ float4 s0[NT], s1[NT], s2[NT], s3[NT];
+  // batches of at most 128 list entries: walks end after ~150 entries at
+  // typical densities, so larger batches would load records nobody reads
+  constexpr int B = NT < 128 ? NT : 128;
+  __shared__ float4 s0[B], s1[B], s2[B], s3[B];
   const uint32_t num = end - start;
   // A pixel is live while !(alpha_sum >= 0.999f) (NaN stays live, as in the
   // reference's test); pixels outside the image start at alpha_sum = 1.
@@ -159,9 +162,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     tr[k] = tl[k] = 1.f;
     walked[k] = 0;
   }
-  for (uint32_t b = 0; b < num; b += NT) {
+  for (uint32_t b = 0; b < num; b += B) {
     const uint32_t it = b + threadIdx.x;
-    if (it < num) {
+    if (threadIdx.x < uint32_t(B) && it < num) {
       float4 q0, q1, q2, q3;
       ld.load(idx2pnt[end - 1 - it], q0, q1, q2, q3);
       s0[threadIdx.x] = q0;
@@ -170,7 +173,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       s3[threadIdx.x] = q3;
     }
     __syncthreads();
-    const uint32_t cnt = min(uint32_t(NT), num - b);
+    const uint32_t cnt = min(uint32_t(B), num - b);
     // Branch-free per pixel: a splat that misses the pixel (or a finished
     // pixel) gets e = 0, which leaves every accumulator bit-identical
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
@@ -337,12 +340,17 @@ struct GradSink {
 
 // LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
 // template instance declaring its own arrays would double the footprint).
-// acc holds one partial-sum slot per wave, so waves combine by plain stores.
+// Batches hold B <= 128 list entries. With one or two waves, acc holds one
+// partial-sum slot per wave (waves combine by plain stores); with more, one
+// slot that the waves add into with LDS float atomics (per-wave slots would
+// take so much LDS that fewer workgroups fit on a CU).
 template <int NT>
 struct BwdLds {
-  float4 s0[NT], s1[NT], s2[NT], s3[NT];
-  uint32_t spid[NT];
-  float acc[NT / 64][kGrad][NT];
+  static constexpr int B = NT < 128 ? NT : 128;
+  static constexpr int S = NT / 64 <= 2 ? NT / 64 : 1;
+  float4 s0[B], s1[B], s2[B], s3[B];
+  uint32_t spid[B];
+  float acc[S][kGrad][B];
   uint32_t smax;
 };
 
@@ -381,7 +389,8 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   uint32_t* spid = L.spid;
   uint32_t& smax = L.smax;
   const uint32_t wv = GM::wave();
-  float(&acc)[kGrad][NT] = L.acc[wv];
+  constexpr int B = BwdLds<NT>::B, S = BwdLds<NT>::S;
+  float(&acc)[kGrad][B] = L.acc[S == 1 ? 0 : wv];
   const int lane = threadIdx.x & 63;
   float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
   // gb = g . (colour behind) + ga * (alpha behind): the back accumulators
@@ -429,7 +438,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   __syncthreads();
   const uint32_t maxw = smax;
   for (uint32_t hi = maxw; hi > 0;) {
-    const uint32_t lo = hi > uint32_t(NT) ? hi - NT : 0u;
+    const uint32_t lo = hi > uint32_t(B) ? hi - B : 0u;
     const uint32_t cnt = hi - lo;
     if (threadIdx.x < cnt) {
       const uint32_t it = hi - 1 - threadIdx.x;
@@ -441,10 +450,15 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       s3[threadIdx.x] = r[3];
       spid[threadIdx.x] = p;
     }
-    // this wave's partial slot: zero for the entries it skips
-    for (uint32_t j = lane; j < cnt; j += 64) {
+    // partial-sum slots start at zero (entries a wave skips stay zero)
+    if (S > 1) {
+      for (uint32_t j = lane; j < cnt; j += 64) {
 #pragma unroll
-      for (int c = 0; c < kGrad; ++c) acc[c][j] = 0.f;
+        for (int c = 0; c < kGrad; ++c) acc[c][j] = 0.f;
+      }
+    } else if (threadIdx.x < cnt) {
+#pragma unroll
+      for (int c = 0; c < kGrad; ++c) acc[c][threadIdx.x] = 0.f;
     }
     __syncthreads();
     for (uint32_t j = 0; j < cnt; ++j) {
@@ -518,8 +532,13 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const float tot = wave_sum8_scatter(v8, lane);
       float vb[1] = {v[8]};
       wave_sums_to_last(vb);
-      if (lane < 8) acc[lane][j] = tot;
-      if (lane == 63) acc[8][j] = vb[0];
+      if (W == 1 || S > 1) {
+        if (lane < 8) acc[lane][j] = tot;
+        if (lane == 63) acc[8][j] = vb[0];
+      } else {
+        if (lane < 8) atomicAdd(&acc[lane][j], tot);
+        if (lane == 63) atomicAdd(&acc[8][j], vb[0]);
+      }
     }
     __syncthreads();
     if (threadIdx.x < cnt) {
@@ -529,7 +548,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       for (int c = 0; c < kGrad; ++c) {
         sum[c] = L.acc[0][c][t];
 #pragma unroll
-        for (int w = 1; w < W; ++w) sum[c] += L.acc[w][c][t];
+        for (int w = 1; w < S; ++w) sum[c] += L.acc[w][c][t];
       }
       const uint32_t p = spid[t];
       const uint32_t g = p - gauss_base;

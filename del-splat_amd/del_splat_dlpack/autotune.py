@@ -51,7 +51,7 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
         return tuple(e["best"]), dict(e["ms"])
     xyz, ls, q, op, rgb = tensors
     fwd_opts = {8: (64,), 16: (64, 128, 256), 32: (128, 256, 512, 1024)}
-    bwd_opts = {8: (64,), 16: (64, 128, 256), 32: (64, 128, 256)}
+    bwd_opts = {8: (64,), 16: (64, 128, 256), 32: (128, 256, 512, 1024)}
 
     def timed(fn):
         fn()

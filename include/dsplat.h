@@ -168,8 +168,8 @@ int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
  * back-to-front from pix2tlast. ADDS into d_pnt2pixxy [N,2], d_pnt2conic
  * [N,3], d_pnt2opacity [N], d_pnt2rgb [N,3]. d_pix2alpha may be null.
  * `threads` selects the workgroup size per tile (0 = default; ts 16:
- * 64/128/256, default 64; ts 32: 64/128/256, default 256; ts 8: always
- * 64). */
+ * 64/128/256, default 64; ts 32: 64/128/256/512/1024, default 256; ts 8:
+ * always 64). */
 int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                const uint32_t* tile2idx, const uint32_t* idx2pnt,

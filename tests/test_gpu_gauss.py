@@ -303,7 +303,7 @@ def test_backward_matches_oracle_adjoint(oracle, ts, with_alpha):
 
 
 @pytest.mark.parametrize("ts,threads", [(16, 64), (16, 128), (16, 256), (32, 64), (32, 128),
-                                        (32, 256), (8, 64)])
+                                        (32, 256), (32, 512), (32, 1024), (8, 64)])
 @pytest.mark.parametrize("with_alpha", [False, True])
 def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
     """The adjoint raster at every workgroup shape (pixel groups per wave,
