@@ -114,6 +114,11 @@ def headline(args, torch, np, SG, scene):
         args.tile = ts
         args.autotune = tune
     stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
+    # tile schedule: each frame's forward is ordered by the previous frame's
+    # per-tile walks (a hint; raster order on the first frame), its adjoint by
+    # its own; results do not depend on the order
+    tw, th = (shape[0] + ts - 1) // ts, (shape[1] + ts - 1) // ts
+    hint = [None]
     ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for k in stages}
     acc = {k: 0.0 for k in stages}
@@ -129,14 +134,18 @@ def headline(args, torch, np, SG, scene):
         t2i, i2p = SG.bin_gauss(shape, ts, pxd, aabb)
         mark("bin", False)
         mark("raster_fwd", True)
-        img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads)
+        cost = torch.empty(tw * th, device=dev, dtype=torch.int32)
+        img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads,
+                                                    tile_order=hint[0], tile_cost=cost)
+        order = SG.tile_order(cost)
+        hint[0] = order
         mark("raster_fwd", False)
         mark("loss_grad", True)
         d_img = 2.0 * (img - target)
         mark("loss_grad", False)
         mark("raster_bwd", True)
         dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast,
-                                                 d_img, None, args.threads_bwd)
+                                                 d_img, None, args.threads_bwd, tile_order=order)
         mark("raster_bwd", False)
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)

@@ -145,28 +145,44 @@ zero_f32_kernel(size_t n, float* __restrict__ p) {
 }
 
 // -------------------------------------------------------------- raster ---
+// Next work item of a persistent workgroup (0, 1, 2, ... across the grid).
+// Wave 0 draws the ticket (lane 0 adds 1, the other lanes add 0) and all its
+// lanes store it. Every branch is on the uniform wave index, so the compiler
+// cannot split a wave at the loop head: with a per-lane `threadIdx.x == 0`
+// fetch it may merge that branch with a per-lane one at the end of the loop
+// body into an inner loop that lane 0 leaves while the other lanes never do.
+__device__ inline uint32_t next_work_item(uint32_t* __restrict__ counter) {
+  __shared__ uint32_t s_item;
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
+    const uint32_t t = atomicAdd(counter, threadIdx.x == 0 ? 1u : 0u);
+    s_item = __builtin_amdgcn_readfirstlane(t);
+  }
+  __syncthreads();
+  const uint32_t q = s_item;
+  __syncthreads();
+  return q;
+}
+
 template <int TS, int NT>
 __global__ void __launch_bounds__(NT)
 mv_raster_fwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t total_tiles,
                      const float4* __restrict__ rec, const uint32_t* __restrict__ tile2idx,
                      const uint32_t* __restrict__ idx2pnt, float* __restrict__ rgb,
                      float* __restrict__ alpha, uint32_t* __restrict__ nwalk,
-                     float* __restrict__ tlast, uint32_t* __restrict__ counter) {
-  __shared__ uint32_t s_item;
+                     float* __restrict__ tlast, uint32_t* __restrict__ counter,
+                     const uint32_t* __restrict__ tile_order, uint32_t* __restrict__ tile_cost) {
   const PackedLoader ld{rec};
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t g = s_item;
-    __syncthreads();
-    if (g >= total_tiles) break;
+    const uint32_t q = next_work_item(counter);
+    if (q >= total_tiles) break;
+    const uint32_t g = tile_order[q];  // longest walks first
     const uint32_t v = view_of_tile(views, nview, g);
     const ViewDev& vd = views[v];
     const uint32_t local = g - vd.tile_off;
     const size_t po = vd.pix_off;
     raster_tile_fwd<TS, NT, PackedLoader>(local % vd.tw, local / vd.tw, vd.w, vd.h, tile2idx[g],
                                           tile2idx[g + 1], ld, idx2pnt, rgb + 3 * po, alpha + po,
-                                          nwalk + po, tlast + po);
+                                          nwalk + po, tlast + po, tile_cost, g);
   }
 }
 
@@ -178,14 +194,12 @@ mv_raster_bwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
                      const uint32_t* __restrict__ nwalk, const float* __restrict__ tlast,
                      const float* __restrict__ d_img, const float* __restrict__ rgb,
                      const float* __restrict__ target, const float* __restrict__ d_alpha,
-                     GradSink out, uint32_t* __restrict__ counter) {
-  __shared__ uint32_t s_item;
+                     GradSink out, uint32_t* __restrict__ counter,
+                     const uint32_t* __restrict__ tile_order) {
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t g = s_item;
-    __syncthreads();
-    if (g >= total_tiles) break;
+    const uint32_t q = next_work_item(counter);
+    if (q >= total_tiles) break;
+    const uint32_t g = tile_order[q];  // longest walks first
     const uint32_t v = view_of_tile(views, nview, g);
     const ViewDev& vd = views[v];
     const uint32_t local = g - vd.tile_off;
@@ -309,6 +323,10 @@ struct dsplat_mv_plan {
   float *rgb = nullptr, *alpha = nullptr, *tlast = nullptr;
   uint32_t* nwalk = nullptr;
   float *d_pixxy = nullptr, *d_conic = nullptr;
+  // per global tile: the last forward's longest walk, and the tiles ordered by
+  // it (the schedule of the next adjoint and forward; raster order after
+  // set_views)
+  uint32_t *tile_cost = nullptr, *tile_order = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap_stream = nullptr;
 };
@@ -383,6 +401,7 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   const size_t o_rgb = take(max_pixels * 12), o_alpha = take(max_pixels * 4);
   const size_t o_nwalk = take(max_pixels * 4), o_tlast = take(max_pixels * 4);
   const size_t o_dpix = take(items * 8), o_dcon = take(items * 12);
+  const size_t o_tcost = take(size_t(P->max_tiles) * 4), o_tord = take(size_t(P->max_tiles) * 4);
   P->mem_bytes = o;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&P->mem), P->mem_bytes);
   if (e != hipSuccess) {
@@ -409,6 +428,8 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   P->tlast = reinterpret_cast<float*>(b + o_tlast);
   P->d_pixxy = reinterpret_cast<float*>(b + o_dpix);
   P->d_conic = reinterpret_cast<float*>(b + o_dcon);
+  P->tile_cost = reinterpret_cast<uint32_t*>(b + o_tcost);
+  P->tile_order = reinterpret_cast<uint32_t*>(b + o_tord);
   // persistent grids: resident workgroups per CU times CUs
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -435,6 +456,7 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
   if (int rc = fill_views(P, num_view, views)) return rc;
   DSPLAT_HIP(hipMemcpyAsync(P->d_views, P->host_views.data(), sizeof(ViewDev) * num_view,
                             hipMemcpyHostToDevice, as_stream(stream)));
+  if (int rc = iota_u32(as_stream(stream), P->total_tiles, P->tile_order)) return rc;
   DSPLAT_HIP(hipStreamSynchronize(as_stream(stream)));
   return DSPLAT_OK;
 }
@@ -479,7 +501,7 @@ template <int TS, int NT>
 static void launch_mv_fwd(dsplat_mv_plan* P, hipStream_t s, float* out_rgb, float* out_alpha) {
   mv_raster_fwd_kernel<TS, NT><<<P->persist_fwd, NT, 0, s>>>(
       P->d_views, P->nview, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, out_rgb, out_alpha,
-      P->nwalk, P->tlast, P->misc + 2);
+      P->nwalk, P->tlast, P->misc + 2, P->tile_order, P->tile_cost);
 }
 
 template <int TS, int NT>
@@ -488,7 +510,7 @@ static void launch_mv_bwd(dsplat_mv_plan* P, hipStream_t s, const float* out_rgb
                           GradSink out) {
   mv_raster_bwd_kernel<TS, NT><<<P->persist_bwd, NT, 0, s>>>(
       P->d_views, P->nview, P->n, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, P->nwalk,
-      P->tlast, d_img, out_rgb, target, d_alpha, out, P->misc + 3);
+      P->tlast, d_img, out_rgb, target, d_alpha, out, P->misc + 3, P->tile_order);
 }
 
 static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
@@ -544,7 +566,8 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   else if (th == 512) launch_mv_fwd<32, 512>(P, s, out_rgb, out_alpha);
   else launch_mv_fwd<32, 256>(P, s, out_rgb, out_alpha);
   DSPLAT_HIP(hipGetLastError());
-  return DSPLAT_OK;
+  // longest walks first for the adjoint and the next forward
+  return order_by_cost(s, P->total_tiles, P->tile_cost, P->tile_order);
 }
 
 static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,

@@ -386,6 +386,58 @@ int fill_f32(hipStream_t s, uint32_t n, float* buf, float v) {
   return DSPLAT_OK;
 }
 
+// ------------------------------------------------------ order by cost ----
+constexpr uint32_t kCostBins = 2048;
+constexpr uint32_t kOrderBlock = 1024;
+
+__global__ void __launch_bounds__(kOrderBlock)
+order_by_cost_kernel(uint32_t n, const uint32_t* __restrict__ cost, uint32_t* __restrict__ order) {
+  __shared__ uint32_t bin[kCostBins];
+  __shared__ uint32_t wsum[kOrderBlock / 64];
+  for (uint32_t i = threadIdx.x; i < kCostBins; i += kOrderBlock) bin[i] = 0;
+  __syncthreads();
+  // bin 0 holds the largest costs
+  for (uint32_t t = threadIdx.x; t < n; t += kOrderBlock)
+    atomicAdd(&bin[kCostBins - 1 - min(cost[t], kCostBins - 1)], 1u);
+  __syncthreads();
+  // exclusive scan of the bins, two per thread
+  const uint32_t a = bin[2 * threadIdx.x], b = bin[2 * threadIdx.x + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_inclusive_scan(a + b);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int k = 0; k < w; ++k) pre += wsum[k];
+  const uint32_t ex = pre + inc - (a + b);
+  bin[2 * threadIdx.x] = ex;
+  bin[2 * threadIdx.x + 1] = ex + a;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n; t += kOrderBlock) {
+    const uint32_t pos = atomicAdd(&bin[kCostBins - 1 - min(cost[t], kCostBins - 1)], 1u);
+    order[pos] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) iota_u32_kernel(uint32_t n, uint32_t* __restrict__ order) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    order[i] = i;
+}
+
+int order_by_cost(hipStream_t s, uint32_t n, const uint32_t* cost, uint32_t* order) {
+  static_assert(kCostBins == 2 * kOrderBlock, "two bins per thread");
+  if (n == 0) return DSPLAT_OK;
+  order_by_cost_kernel<<<1, kOrderBlock, 0, s>>>(n, cost, order);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+int iota_u32(hipStream_t s, uint32_t n, uint32_t* order) {
+  if (n == 0) return DSPLAT_OK;
+  iota_u32_kernel<<<min(div_up(n, 256), 1024u), 256, 0, s>>>(n, order);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
 }  // namespace dsplat
 
 // ------------------------------------------------------------------ C ABI ---
@@ -430,4 +482,11 @@ extern "C" int dsplat_radix_sort_by_key_u64_u32(void* stream, uint32_t n, uint64
 extern "C" int dsplat_fill_f32(void* stream, uint32_t n, float* buf, float value) {
   if (n && !buf) return dsplat_set_error(DSPLAT_ERR_ARG, "fill: null buffer");
   return fill_f32(as_stream(stream), n, buf, value);
+}
+
+extern "C" int dsplat_tile_order(void* stream, uint32_t num_tile, const uint32_t* tile_cost,
+                                 uint32_t* tile_order) {
+  if (num_tile && (!tile_cost || !tile_order))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "tile_order: null buffer");
+  return order_by_cost(as_stream(stream), num_tile, tile_cost, tile_order);
 }

@@ -34,4 +34,12 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
 
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 
+// order[0..n) = 0..n-1 by descending cost (ties in any order): a counting
+// sort over min(cost, 2047) in one workgroup. A schedule, not a result:
+// kernels that take it produce the same output for any order.
+int order_by_cost(hipStream_t s, uint32_t n, const uint32_t* cost, uint32_t* order);
+
+// order[i] = i.
+int iota_u32(hipStream_t s, uint32_t n, uint32_t* order);
+
 }  // namespace dsplat

@@ -130,7 +130,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                                                 float* __restrict__ pix2rgb,
                                                 float* __restrict__ pix2alpha,
                                                 uint32_t* __restrict__ pix2nwalk,
-                                                float* __restrict__ pix2tlast) {
+                                                float* __restrict__ pix2tlast,
+                                                uint32_t* __restrict__ tile_cost = nullptr,
+                                                uint32_t cost_slot = 0) {
 #pragma clang fp contract(off)
   using GM = GroupMap<TS, NT>;
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q;
@@ -247,6 +249,30 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     if (pix2alpha) pix2alpha[ip] = asum[k];
     if (pix2nwalk) pix2nwalk[ip] = walked[k];
     if (pix2tlast) pix2tlast[ip] = tl[k];
+  }
+  if (tile_cost) {
+    // The tile's longest walk: what its adjoint (and the next frame's
+    // forward) will cost, used to schedule heavy tiles first. Every branch
+    // here is on the (uniform) wave index and every lane of a wave stores
+    // the same value: in a persistent loop, a per-lane `threadIdx.x == 0`
+    // store here and the per-lane work fetch at the loop head were merged by
+    // the compiler into an inner loop that lane 0 left while the other lanes
+    // kept re-running the same tile.
+    constexpr int W = NT / 64;
+    __shared__ uint32_t s_wmax[W];
+    uint32_t w = 0;
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) w = max(w, walked[k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) w = max(w, uint32_t(__shfl_xor(int(w), off, 64)));
+    s_wmax[GM::wave()] = __builtin_amdgcn_readfirstlane(w);
+    __syncthreads();
+    if (GM::wave() == 0) {
+      uint32_t m = s_wmax[0];
+#pragma unroll
+      for (int i = 1; i < W; ++i) m = max(m, s_wmax[i]);
+      tile_cost[cost_slot] = m;
+    }
   }
 }
 
@@ -432,7 +458,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, uint32_t(__shfl_xor(int(wmax), off, 64)));
   wmax = __builtin_amdgcn_readfirstlane(wmax);
-  if (threadIdx.x == 0) smax = 0;
+  if (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) smax = 0;  // wave 0, uniform branch
   __syncthreads();
   if (lane == 0) atomicMax(&smax, wmax);
   __syncthreads();

@@ -105,8 +105,11 @@ def project_fwd(xyz, scale, quat, opacity, rgb, m16, img_shape, log_scale=True):
     return pxd, conic, aabb, splat2
 
 
-def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0):
-    """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tlast [H,W])."""
+def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0, tile_order=None,
+                  tile_cost=None):
+    """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tlast [H,W]).
+    tile_order (int32 [T], a permutation) schedules the tiles (results do not
+    depend on it); tile_cost (int32 [T]) receives each tile's longest walk."""
     dev = splat2.device
     w, h = img_shape
     img = torch.empty((h, w, 3), device=dev, dtype=torch.float32)
@@ -114,15 +117,37 @@ def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0):
     nwalk = torch.empty((h, w), device=dev, dtype=torch.int32)
     tlast = torch.empty((h, w), device=dev, dtype=torch.float32)
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
-    call("dsplat_gauss_rasterize_fwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
-         ptr(tile2idx), ptr(i2p), ptr(img), ptr(alpha), ptr(nwalk), ptr(tlast))
+    if tile_order is None and tile_cost is None:
+        call("dsplat_gauss_rasterize_fwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
+             ptr(tile2idx), ptr(i2p), ptr(img), ptr(alpha), ptr(nwalk), ptr(tlast))
+    else:
+        _check_schedule(tile2idx, tile_order, tile_cost)
+        call("dsplat_gauss_rasterize_fwd_ordered", stream_ptr(dev), w, h, tile_size, threads,
+             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(img), ptr(alpha),
+             ptr(nwalk), ptr(tlast), ptr(tile_cost))
     return img, alpha, nwalk, tlast
 
 
+def _check_schedule(tile2idx, *arrays):
+    for a in arrays:
+        if a is not None and (a.dtype != torch.int32 or a.numel() != tile2idx.numel() - 1):
+            raise ValueError("tile schedule arrays must be int32 with one entry per tile")
+
+
+def tile_order(tile_cost):
+    """Tiles by descending cost (int32 [T]): the schedule for rasterize_bwd and,
+    as a hint, the next frame's rasterize_fwd."""
+    order = torch.empty_like(tile_cost)
+    call("dsplat_tile_order", stream_ptr(tile_cost.device), tile_cost.numel(), ptr(tile_cost),
+         ptr(order))
+    return order
+
+
 def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, d_img,
-                  d_alpha=None, threads=0):
+                  d_alpha=None, threads=0, tile_order=None):
     """Returns 2D gradients (d_pixxy [N,2], d_conic [N,3], d_opacity [N],
-    d_rgb [N,3])."""
+    d_rgb [N,3]). tile_order as for rasterize_fwd (tile_order(tile_cost) of
+    the forward is the natural one)."""
     dev = splat2.device
     n = splat2.shape[0]
     w, h = img_shape
@@ -134,9 +159,15 @@ def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast,
     d_rgb = acc[6 * n:].view(n, 3)
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
     da = None if d_alpha is None else d_alpha.contiguous()
-    call("dsplat_gauss_rasterize_bwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
-         ptr(tile2idx), ptr(i2p), ptr(nwalk), ptr(tlast), ptr(d_img.contiguous()), ptr(da),
-         ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
+    if tile_order is None:
+        call("dsplat_gauss_rasterize_bwd", stream_ptr(dev), w, h, tile_size, threads,
+             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(nwalk), ptr(tlast),
+             ptr(d_img.contiguous()), ptr(da), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
+    else:
+        _check_schedule(tile2idx, tile_order)
+        call("dsplat_gauss_rasterize_bwd_ordered", stream_ptr(dev), w, h, tile_size, threads,
+             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tlast),
+             ptr(d_img.contiguous()), ptr(da), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
     return d_pixxy, d_conic, d_op, d_rgb
 
 
@@ -158,6 +189,10 @@ def bin_gauss(img_shape, tile_size, pxd, aabb):
     return tile2idx_idx2pnt(img_shape, tile_size, pxd, aabb)
 
 
+# last tile order per (device, image shape, tile size): a scheduling hint only
+_SCHEDULE = {}
+
+
 class _Render(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, log_scale, quat, opacity, rgb, m16, img_shape, tile_size, threads,
@@ -167,9 +202,18 @@ class _Render(torch.autograd.Function):
         pxd, conic, aabb, splat2 = project_fwd(xyz, log_scale, quat, opacity, rgb, m16,
                                                img_shape, log_scale=True)
         tile2idx, idx2pnt = bin_gauss(img_shape, tile_size, pxd, aabb)
+        # schedule: the previous frame's order at this image / tile size as a
+        # hint for the forward, this frame's walks for the adjoint
+        key = (str(xyz.device), tuple(img_shape), tile_size)
+        hint = _SCHEDULE.get(key)
+        cost = torch.empty(tile2idx.numel() - 1, device=xyz.device, dtype=torch.int32)
         img, alpha, nwalk, tlast = rasterize_fwd(img_shape, tile_size, splat2, tile2idx,
-                                                 idx2pnt, threads)
-        ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast)
+                                                 idx2pnt, threads, tile_order=hint,
+                                                 tile_cost=cost)
+        order = tile_order(cost)
+        _SCHEDULE[key] = order
+        ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast,
+                              order)
         ctx.img_shape = img_shape
         ctx.tile_size = tile_size
         ctx.threads = threads_bwd
@@ -178,12 +222,13 @@ class _Render(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, d_img, d_alpha):
-        xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast = ctx.saved_tensors
+        (xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast,
+         order) = ctx.saved_tensors
         if d_img is None:
             d_img = torch.zeros((ctx.img_shape[1], ctx.img_shape[0], 3), device=xyz.device)
         d_pixxy, d_conic, d_op, d_rgb = rasterize_bwd(ctx.img_shape, ctx.tile_size, splat2,
                                                       tile2idx, idx2pnt, nwalk, tlast, d_img,
-                                                      d_alpha, ctx.threads)
+                                                      d_alpha, ctx.threads, tile_order=order)
         d_xyz, d_ls, d_quat = project_bwd(xyz, log_scale, quat, m16, ctx.img_shape, d_pixxy,
                                           d_conic, log_scale=True)
         return d_xyz, d_ls, d_quat, d_op, d_rgb, None, None, None, None, None

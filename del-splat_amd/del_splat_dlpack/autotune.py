@@ -44,7 +44,7 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
     threads_bwd), {config: ms})."""
     from .SplatGauss import torch as SG
 
-    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd_v3")
+    sig = _signature(num_pnt, [(None, img_shape)], "single_fwd_bwd_v4")
     db = _load() if use_cache else {}
     if sig in db:
         e = db[sig]
@@ -68,13 +68,18 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
         t_front = timed(lambda: SG.bin_gauss(img_shape, ts, *SG.project_fwd(
             xyz, ls, q, op, rgb, m16, img_shape)[::2]))
         t2i, i2p = SG.bin_gauss(img_shape, ts, pxd, aabb)
-        tf = {th: timed(lambda th=th: SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th))
+        # timed with the walk-ordered tile schedule the render path uses
+        cost = torch.empty(t2i.numel() - 1, device=t2i.device, dtype=torch.int32)
+        img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p,
+                                                    tile_cost=cost)
+        order = SG.tile_order(cost)
+        tf = {th: timed(lambda th=th: SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th,
+                                                       tile_order=order, tile_cost=cost))
               for th in fwd_opts[ts]}
         thf = min(tf, key=tf.get)
-        img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, thf)
         d_img = 2.0 * (img - target)
         tb = {th: timed(lambda th=th: SG.rasterize_bwd(img_shape, ts, splat2, t2i, i2p, nwalk,
-                                                      tlast, d_img, None, th))
+                                                      tlast, d_img, None, th, tile_order=order))
               for th in bwd_opts[ts]}
         thb = min(tb, key=tb.get)
         total = t_front + tf[thf] + tb[thb]

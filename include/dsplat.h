@@ -178,6 +178,31 @@ int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                float* d_pnt2pixxy, float* d_pnt2conic, float* d_pnt2opacity,
                                float* d_pnt2rgb);
 
+/* Tile schedule (no counterpart in the reference; results do not depend on
+ * it). The _ordered raster variants process tile tile_order[i] as their i-th
+ * workgroup (tile_order: u32[T], a permutation of 0..T-1; null = raster
+ * order). dsplat_gauss_rasterize_fwd_ordered also writes tile_cost[t] = the
+ * longest walk (pix2nwalk) in tile t (may be null), and dsplat_tile_order
+ * turns such costs into the descending-cost order. Launching the longest
+ * walks first shortens the tail of a launch: a frame's forward costs order
+ * its adjoint and, as a hint, the next frame's forward. */
+int dsplat_gauss_rasterize_fwd_ordered(void* stream, uint32_t img_w, uint32_t img_h,
+                                       uint32_t tile_size, uint32_t threads,
+                                       const float* pnt2splat2, const uint32_t* tile2idx,
+                                       const uint32_t* idx2pnt, const uint32_t* tile_order,
+                                       float* pix2rgb, float* pix2alpha, uint32_t* pix2nwalk,
+                                       float* pix2tlast, uint32_t* tile_cost);
+int dsplat_gauss_rasterize_bwd_ordered(void* stream, uint32_t img_w, uint32_t img_h,
+                                       uint32_t tile_size, uint32_t threads,
+                                       const float* pnt2splat2, const uint32_t* tile2idx,
+                                       const uint32_t* idx2pnt, const uint32_t* tile_order,
+                                       const uint32_t* pix2nwalk, const float* pix2tlast,
+                                       const float* d_pix2rgb, const float* d_pix2alpha,
+                                       float* d_pnt2pixxy, float* d_pnt2conic,
+                                       float* d_pnt2opacity, float* d_pnt2rgb);
+int dsplat_tile_order(void* stream, uint32_t num_tile, const uint32_t* tile_cost,
+                      uint32_t* tile_order);
+
 /* Adjoint of the projection. ADDS into d_pnt2xyz [N,3], d_pnt2scale [N,3]
  * (w.r.t. sigma, or log sigma under DSPLAT_SCALE_IS_LOG), d_pnt2quat [N,4];
  * with DSPLAT_GRAD_OVERWRITE in flags it stores them instead (no zeroing

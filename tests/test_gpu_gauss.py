@@ -352,3 +352,41 @@ def test_project_bwd_add_and_overwrite():
                  *[a.data_ptr() for a in add])
     for a, b in zip(add, store):
         np.testing.assert_allclose(a.cpu().numpy(), 1.0 + b.cpu().numpy(), rtol=1e-6, atol=1e-6)
+
+
+def test_tile_schedule_orders_and_results():
+    """dsplat_tile_order is a permutation by descending (capped) cost; the
+    forward in any tile order is bit-identical to raster order and reports
+    each tile's longest walk; the adjoint in any order agrees to float
+    atomic reordering."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    rng = np.random.default_rng(3)
+    cost = rng.integers(0, 5000, 9000).astype(np.int32)
+    order = SG.tile_order(_t(cost)).cpu().numpy()
+    assert sorted(order.tolist()) == list(range(9000))
+    capped = np.minimum(cost[order], 2047)
+    assert (np.diff(capped) <= 0).all()
+    shape, ts = (171, 133), 16
+    s, m = small_scene(4000, 91, shape)
+    op, rgb = _t(s["opacity"]), _t(s["rgb"])
+    pxd, conic, aabb, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op,
+                                              rgb, _t(m), shape, log_scale=False)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    T = t2i.numel() - 1
+    ref = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
+    perm = _t(rng.permutation(T).astype(np.int32))
+    tc = torch.zeros(T, device=DEV, dtype=torch.int32)
+    got = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, tile_order=perm, tile_cost=tc)
+    for a, b in zip(ref, got):
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+    nw = ref[2].cpu().numpy()
+    tw, th = (shape[0] + ts - 1) // ts, (shape[1] + ts - 1) // ts
+    pad = np.zeros((th * ts, tw * ts), np.int64)
+    pad[:shape[1], :shape[0]] = nw
+    np.testing.assert_array_equal(tc.cpu().numpy(), pad.reshape(th, ts, tw, ts).max(axis=(1, 3)).ravel())
+    d_img = _t(rng.standard_normal((shape[1], shape[0], 3)).astype(np.float32))
+    g0 = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, ref[2], ref[3], d_img)
+    g1 = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, ref[2], ref[3], d_img,
+                          tile_order=SG.tile_order(tc))
+    for a, b in zip(g0, g1):
+        assert max_rel(b.cpu().numpy(), a.cpu().numpy()) <= 1e-5
