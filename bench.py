@@ -142,10 +142,14 @@ def headline(args, torch, np, SG, scene):
         mark("raster_fwd", False)
         mark("loss_grad", True)
         d_img = 2.0 * (img - target)
+        # the adjoint's accumulators, zeroed here so the raster_bwd events
+        # bracket the adjoint kernel alone
+        acc = torch.zeros(9 * splat2.shape[0], device=dev, dtype=torch.float32)
         mark("loss_grad", False)
         mark("raster_bwd", True)
         dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast,
-                                                 d_img, None, args.threads_bwd, tile_order=order)
+                                                 d_img, None, args.threads_bwd, tile_order=order,
+                                                 grad_acc=acc)
         mark("raster_bwd", False)
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)

@@ -96,6 +96,18 @@ __device__ inline uint32_t wave_incl_max_u32(uint32_t v) {
   return v;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (same shifts and row
+// broadcasts as wave_incl_max_u32).
+__device__ inline uint32_t wave_incl_sum_u32(uint32_t v) {
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xF, 0xF, true));
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xF, 0xF, true));
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xF, 0xF, true));
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xF, 0xF, true));
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xA, 0xF, false));
+  v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xC, 0xF, false));
+  return v;
+}
+
 // Writes the (tile, value) pairs of the calling wave's 64 consecutive items:
 // item l owns output slots [off, off + count) and covers the tile rectangle
 // r (ix-outer, iy-inner, tile id = tbase + iy * tw + ix). Items are valid on

@@ -3,8 +3,8 @@
 // sort_by_key_u64 and array1d::fill used by the reference host layer
 // (del-splat-cudarc/src/splat_gauss.rs:137-146,184-191).
 //
-// Radix sort: 8-bit digits, one (upsweep, column scan, scatter) triple per
-// pass. A 256-thread workgroup owns a 2048-element tile; each wave64 owns 512
+// Radix sort: digits of at most 8 bits (the key bits spread evenly over the
+// fewest passes), one (upsweep, column scan, scatter) triple per pass. A 256-thread workgroup owns a 2048-element tile; each wave64 owns 512
 // consecutive elements and ranks them in rounds of 64 with a ballot-based
 // digit match, keeping per-wave digit counters in LDS, so the rank of an
 // element is (digit offset of earlier tiles) + (count in earlier waves of the
@@ -20,15 +20,8 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
-__device__ inline uint32_t wave_inclusive_scan(uint32_t v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
-  return v;
-}
+// DPP row shifts and broadcasts: no LDS round trips.
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v) { return wave_incl_sum_u32(v); }
 
 // Exclusive scan of one value per thread over a 256-thread block; returns
 // the block total in *total.
@@ -184,10 +177,18 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   __syncthreads();
   const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
   const uint32_t mask = (1u << dbits) - 1u;
+  // all loads first, then the histogram
+  KeyT kk[kSortItems];
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
+    kk[k] = i < n ? keys[i] : KeyT(0);
+  }
+#pragma unroll
   for (int k = 0; k < kSortItems; ++k) {
     const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t d = valid ? uint32_t(keys[i] >> shift) & mask : 0u;
+    const uint32_t d = valid ? uint32_t(kk[k] >> shift) & mask : 0u;
     const uint64_t act = __ballot(valid);
     if (act == 0) continue;  // wave-uniform
     // valid lanes are a prefix of the wave, so lane 0 is one of them; a
@@ -253,6 +254,11 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   const uint64_t bstart = uint64_t(blockIdx.x) * kSortTile;
   if (bstart >= n) return;  // uniform
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the digit total and this tile's column prefix of digit threadIdx.x,
+  // fetched now so their latency hides behind the ranking
+  const bool live_digit = threadIdx.x < nbins;
+  const uint32_t my_tot = live_digit ? tot[threadIdx.x] : 0u;
+  const uint32_t my_off = live_digit ? off[size_t(threadIdx.x) * nb + blockIdx.x] : 0u;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
@@ -265,6 +271,11 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     const bool valid = i < n;
     key[r] = valid ? kin[i] : KeyT(0);
     val[r] = valid ? (iota ? uint32_t(i) : vin[i]) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint64_t i = base + uint64_t(r) * 64 + lane;
+    const bool valid = i < n;
     const uint32_t d = uint32_t(key[r] >> shift) & mask;
     dig[r] = valid ? d : 0xffffffffu;
     uint64_t peers = __ballot(valid);
@@ -294,9 +305,8 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   // earlier tiles
   uint32_t btot, all;
   dpre[threadIdx.x] = block_exclusive_scan(total, &btot);
-  const bool live_digit = threadIdx.x < nbins;
-  const uint32_t gpre = block_exclusive_scan(live_digit ? tot[threadIdx.x] : 0u, &all);
-  if (live_digit) dbase[threadIdx.x] = gpre + off[size_t(threadIdx.x) * nb + blockIdx.x];
+  const uint32_t gpre = block_exclusive_scan(my_tot, &all);
+  if (live_digit) dbase[threadIdx.x] = gpre + my_off;
   __syncthreads();
   // stage in LDS in tile-sorted order ...
 #pragma unroll
@@ -308,16 +318,26 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     sval[lp] = val[r];
   }
   __syncthreads();
-  // ... and write each digit's run to consecutive global addresses
+  // ... and write each digit's run to consecutive global addresses (the
+  // gathered values loaded for all rounds before any is stored)
   const uint32_t nvalid = uint32_t(min(uint64_t(kSortTile), n - bstart));
-  for (uint32_t i = threadIdx.x; i < nvalid; i += kSortBlock) {
-    const KeyT k = skey[i];
-    const uint32_t d = uint32_t(k >> shift) & mask;
-    const uint32_t pos = dbase[d] + (i - dpre[d]);
-    const uint32_t v = sval[i];
-    kout[pos] = k;
-    vout[pos] = v;
-    if (gdst) gdst[pos] = gsrc[v];
+  uint32_t gv[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
+    gv[r] = (gdst && i < nvalid) ? gsrc[sval[i]] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
+    if (i < nvalid) {
+      const KeyT k = skey[i];
+      const uint32_t d = uint32_t(k >> shift) & mask;
+      const uint32_t pos = dbase[d] + (i - dpre[d]);
+      kout[pos] = k;
+      vout[pos] = sval[i];
+      if (gdst) gdst[pos] = gv[r];
+    }
   }
 }
 
@@ -342,12 +362,15 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
   if (vals_alt_out) *vals_alt_out = L.valt;
   if (n_cap == 0) return DSPLAT_OK;
   const uint32_t nb = div_up(n_cap, kSortTile);
+  // as few passes as 8-bit digits allow, with the bits spread evenly over
+  // them: fewer digits per pass give longer runs per digit in the scatter
+  // (11 bits: 6 + 5 rather than 8 + 3)
   const uint32_t P = div_up(key_bits, 8u);
   KeyT* ksrc = keys; uint32_t* vsrc = vals; KeyT* kdst = kalt; uint32_t* vdst = L.valt;
   bool in_alt = false;
+  uint32_t shift = 0;
   for (uint32_t p = 0; p < P; ++p) {
-    const uint32_t shift = 8u * p;
-    const uint32_t dbits = min(8u, key_bits - shift);
+    const uint32_t dbits = div_up(key_bits - shift, P - p);
     radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb,
                                                          L.cnt);
     radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, L.cnt, L.tot);
@@ -358,6 +381,7 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;
+    shift += dbits;
   }
   DSPLAT_HIP(hipGetLastError());
   *result_in_alt = in_alt;

@@ -144,15 +144,23 @@ def tile_order(tile_cost):
 
 
 def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, d_img,
-                  d_alpha=None, threads=0, tile_order=None):
+                  d_alpha=None, threads=0, tile_order=None, grad_acc=None):
     """Returns 2D gradients (d_pixxy [N,2], d_conic [N,3], d_opacity [N],
     d_rgb [N,3]). tile_order as for rasterize_fwd (tile_order(tile_cost) of
-    the forward is the natural one)."""
+    the forward is the natural one). The adjoint adds into its outputs:
+    grad_acc, if given, is the caller's float32 [9N] buffer, already zeroed,
+    that the four outputs are views of; otherwise one is allocated zeroed."""
     dev = splat2.device
     n = splat2.shape[0]
     w, h = img_shape
-    # the adjoint accumulates: one zero fill for all four
-    acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
+    if grad_acc is None:
+        acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
+    else:
+        if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
+                or not grad_acc.is_contiguous() or grad_acc.device != dev):
+            raise ValueError("grad_acc must be a contiguous float32 [9N] tensor on the "
+                             "splats' device")
+        acc = grad_acc.view(-1)
     d_pixxy = acc[:2 * n].view(n, 2)
     d_conic = acc[2 * n:5 * n].view(n, 3)
     d_op = acc[5 * n:6 * n]

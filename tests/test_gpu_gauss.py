@@ -126,6 +126,40 @@ def test_binning_one_call_and_count_only(oracle):
     assert i2p.numel() == 0 and int(t2i.abs().sum().item()) == 0
 
 
+@pytest.mark.parametrize("n,shape", [(5000, (64, 64)), (20000, (64, 64)), (20000, (40, 30))])
+def test_binning_dense_tiles_and_equal_depths(oracle, n, shape):
+    """Every box covers the whole image, so each tile's list holds all n
+    Gaussians: n = 5000 sorts each tile in LDS, n = 20000 exceeds the LDS
+    segment capacity (8192) and streams; a third of the depths repeat three
+    values, so the equal-key path (index order within a depth) runs too."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    rng = np.random.default_rng(n + shape[0])
+    boxes = np.tile(np.array([[-1.0, -1.0, shape[0] + 1.0, shape[1] + 1.0]], np.float32), (n, 1))
+    boxes[::7, 0] = 10.0  # some boxes start one tile in
+    pxd = np.zeros((n, 3), np.float32)
+    pxd[:, 2] = rng.uniform(-0.9, 0.9, n).astype(np.float32)
+    pxd[::3, 2] = np.float32([0.1, 0.2, 0.3])[rng.integers(0, 3, len(pxd[::3]))]
+    for ts in (16, 32):
+        t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, _t(pxd), _t(boxes))
+        o_t2i, o_i2p = oracle.tile2pnt(boxes, pxd, shape, ts)
+        np.testing.assert_array_equal(u32(t2i), o_t2i)
+        np.testing.assert_array_equal(u32(i2p), o_i2p)
+
+
+def test_binning_large_tile_grid(oracle):
+    """A 1024x1024 image at 8-pixel tiles (16384 tiles) is binned by the
+    general path (depth sort, pair fill, tile sort); same lists as the
+    oracle."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (1024, 1024)
+    s, m = small_scene(20000, 8, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, 8, pxd, aabb)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, 8)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+
+
 def _copy_device(dst, src, nbytes):
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
