@@ -112,11 +112,31 @@ struct GroupMap {
   }
 };
 
-// Whether the box (x0, y0, x1, y1) can contain a pixel centre of the region
-// [cx0, cx1] x [cy0, cy1] (centres).
-__device__ inline bool region_hit(float bx0, float by0, float bx1, float by1, float cx0,
-                                  float cy0, float cx1, float cy1) {
-  return !(bx1 < cx0 || bx0 > cx1 || by1 < cy0 || by0 > cy1);
+// The tile's regions whose pixel-centre rectangle [cx0, cx1] x [cy0, cy1]
+// the box (bx0, by0, bx1, by1) can reach, !(bx1 < cx0 || bx0 > cx1 || by1 <
+// cy0 || by0 > cy1), as bits ry * RXN + rx (GroupMap's numbering: group g of
+// wave w is region w * G + g). One thread evaluates it per list entry while
+// staging the entry (the test is separable into x and y); the waves then
+// test a bit instead of comparing the box against their regions in every
+// lane.
+template <int TS, int NT>
+__device__ inline uint32_t region_mask(uint32_t tx, uint32_t ty, float bx0, float by0, float bx1,
+                                       float by1) {
+  using GM = GroupMap<TS, NT>;
+  constexpr int RXN = GM::RXN, RS = GM::RS;
+  uint32_t xm = 0, ym = 0;
+#pragma unroll
+  for (int r = 0; r < RXN; ++r) {
+    const float cx0 = float(tx * TS + uint32_t(r * RS)) + 0.5f;
+    const float cy0 = float(ty * TS + uint32_t(r * RS)) + 0.5f;
+    const float cx1 = cx0 + float(RS - 1), cy1 = cy0 + float(RS - 1);
+    xm |= uint32_t(!(bx1 < cx0 || bx0 > cx1)) << r;
+    ym |= uint32_t(!(by1 < cy0 || by0 > cy1)) << r;
+  }
+  uint32_t m = 0;
+#pragma unroll
+  for (int r = 0; r < RXN; ++r) m |= ((ym >> r) & 1u) ? xm << (r * RXN) : 0u;
+  return m;
 }
 
 // Forward of one tile. Image arrays are the view's base pointers (pixel
@@ -140,20 +160,13 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
   // typical densities, so larger batches would load records nobody reads
   constexpr int B = NT < 128 ? NT : 128;
   __shared__ float4 s0[B], s1[B], s2[B], s3[B];
+  __shared__ uint32_t smask[B];
   const uint32_t num = end - start;
+  const uint32_t gshift = GM::wave() * uint32_t(G), gbits = (1u << G) - 1u;
   // A pixel is live while !(alpha_sum >= 0.999f) (NaN stays live, as in the
   // reference's test); pixels outside the image start at alpha_sum = 1.
   float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
   uint32_t walked[PPT];
-  // pixel-centre bounds of each group's region (wave-uniform)
-  float gx0[G], gy0[G], gx1[G], gy1[G];
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    gx0[g] = float(tx * TS + GM::rx0(g)) + 0.5f;
-    gy0[g] = float(ty * TS + GM::ry0(g)) + 0.5f;
-    gx1[g] = gx0[g] + float(GM::RS - 1);
-    gy1[g] = gy0[g] + float(GM::RS - 1);
-  }
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
     const uint32_t ix = tx * TS + GM::lx(k), iy = ty * TS + GM::ly(k);
@@ -173,6 +186,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       s1[threadIdx.x] = q1;
       s2[threadIdx.x] = q2;
       s3[threadIdx.x] = q3;
+      smask[threadIdx.x] = region_mask<TS, NT>(tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     __syncthreads();
     const uint32_t cnt = min(uint32_t(B), num - b);
@@ -180,6 +194,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     // pixel) gets e = 0, which leaves every accumulator bit-identical
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
     for (uint32_t j = 0; j < cnt; ++j) {
+      // groups of this wave that the box reaches (scalar)
+      const uint32_t hits = (__builtin_amdgcn_readfirstlane(smask[j]) >> gshift) & gbits;
+      if (hits == 0) continue;
       bool live[PPT];
       bool anylive = false;
 #pragma unroll
@@ -190,20 +207,12 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       if (__ballot(anylive) == 0) break;  // every pixel of the wave is done
       const float4 q2 = s2[j];
       const float4 q3 = s3[j];
-      bool hit[G];
-      bool anyg = false;
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        hit[g] = region_hit(q2.y, q2.z, q2.w, q3.x, gx0[g], gy0[g], gx1[g], gy1[g]);
-        anyg = anyg || hit[g];
-      }
-      if (!anyg) continue;  // the box reaches none of the wave's regions
       const float4 q0 = s0[j];
       const float4 q1 = s1[j];
       const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        if (!hit[g]) continue;  // wave-uniform
+        if (!((hits >> g) & 1u)) continue;  // wave-uniform
         bool inm[Q];
         bool any = false;
 #pragma unroll
@@ -377,6 +386,7 @@ struct BwdLds {
   float4 s0[B], s1[B], s2[B], s3[B];
   uint32_t spid[B];
   float acc[S][kGrad][B];
+  uint32_t smask[B];  // region_mask of each staged entry
   uint32_t smax;
 };
 
@@ -399,22 +409,15 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
 #pragma clang fp contract(off)
   using GM = GroupMap<TS, NT>;
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q, W = GM::W;
-  // pixel-centre bounds of each group's region (wave-uniform)
-  float gx0[G], gy0[G], gx1[G], gy1[G];
-#pragma unroll
-  for (int g = 0; g < G; ++g) {
-    gx0[g] = float(tx * TS + GM::rx0(g)) + 0.5f;
-    gy0[g] = float(ty * TS + GM::ry0(g)) + 0.5f;
-    gx1[g] = gx0[g] + float(GM::RS - 1);
-    gy1[g] = gy0[g] + float(GM::RS - 1);
-  }
   float4* s0 = L.s0;
   float4* s1 = L.s1;
   float4* s2 = L.s2;
   float4* s3 = L.s3;
   uint32_t* spid = L.spid;
+  uint32_t* smask = L.smask;
   uint32_t& smax = L.smax;
   const uint32_t wv = GM::wave();
+  const uint32_t gshift = wv * uint32_t(G), gbits = (1u << G) - 1u;
   constexpr int B = BwdLds<NT>::B, S = BwdLds<NT>::S;
   float(&acc)[kGrad][B] = L.acc[S == 1 ? 0 : wv];
   const int lane = threadIdx.x & 63;
@@ -470,11 +473,13 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const uint32_t it = hi - 1 - threadIdx.x;
       const uint32_t p = idx2pnt[end - 1 - it];
       const float4* r = splat2 + 4 * size_t(p);
+      const float4 q2 = r[2], q3 = r[3];
       s0[threadIdx.x] = r[0];
       s1[threadIdx.x] = r[1];
-      s2[threadIdx.x] = r[2];
-      s3[threadIdx.x] = r[3];
+      s2[threadIdx.x] = q2;
+      s3[threadIdx.x] = q3;
       spid[threadIdx.x] = p;
+      smask[threadIdx.x] = region_mask<TS, NT>(tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     // partial-sum slots start at zero (entries a wave skips stay zero)
     if (S > 1) {
@@ -490,24 +495,20 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     for (uint32_t j = 0; j < cnt; ++j) {
       const uint32_t it = hi - 1 - j;
       if (it >= wmax) continue;  // beyond every walk of the wave
+      // groups of this wave that the box reaches (scalar)
+      const uint32_t hits = (__builtin_amdgcn_readfirstlane(smask[j]) >> gshift) & gbits;
+      if (hits == 0) continue;
       const float4 q2 = s2[j], q3 = s3[j];
-      bool hit[G];
-      bool anyg = false;
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        hit[g] = region_hit(q2.y, q2.z, q2.w, q3.x, gx0[g], gy0[g], gx1[g], gy1[g]);
-        anyg = anyg || hit[g];
-      }
-      if (!anyg) continue;  // the box reaches none of the wave's regions
       const float4 q0 = s0[j], q1 = s1[j];
       const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      // -0 so that the first group's terms enter without an add (x + -0 = x)
       float v[kGrad];
 #pragma unroll
-      for (int c = 0; c < kGrad; ++c) v[c] = 0.f;
+      for (int c = 0; c < kGrad; ++c) v[c] = -0.f;
       bool touched = false;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        if (!hit[g]) continue;  // wave-uniform
+        if (!((hits >> g) & 1u)) continue;  // wave-uniform
         bool inm[Q];
         bool any = false;
 #pragma unroll
@@ -542,10 +543,11 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           v[8] = fmaf(g2[k], et, v[8]);
           gb[k] = fmaf(gc, et, gb[k]);
           v[5] = fmaf(dl_de, gk, v[5]);
-          // moments of dL/dq: the mean and conic gradients of the entry are
-          // linear in (S_x, S_y, S_xx, S_xy, S_yy) and formed once at the flush
-          const float dl_dq = (dl_de * e) * -0.5f;
-          const float a = dl_dq * dx, bq = dl_dq * dy;
+          // moments of -2 dL/dq = dL/de e: the mean and conic gradients of
+          // the entry are linear in (S_x, S_y, S_xx, S_xy, S_yy), formed once
+          // at the flush (where the factor -1/2 is applied, exactly)
+          const float dl_dq2 = dl_de * e;
+          const float a = dl_dq2 * dx, bq = dl_dq2 * dy;
           v[0] = v[0] + a;
           v[1] = v[1] + bq;
           v[2] = fmaf(a, dx, v[2]);
@@ -579,12 +581,14 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const uint32_t p = spid[t];
       const uint32_t g = p - gauss_base;
       const float ca = s0[t].z, cb = s0[t].w, cc = s1[t].x;
-      // d q / d mean = -2 C d  ->  dL/dmean = -2 C (S_x, S_y)
-      const float a0 = -2.f * (ca * sum[0] + cb * sum[1]), a1 = -2.f * (cb * sum[0] + cc * sum[1]);
+      // sum[0..4] are moments of -2 dL/dq. d q / d mean = -2 C d, so
+      // dL/dmean = -2 C (S_x, S_y) = C (sum[0], sum[1]) (scaling by powers
+      // of two is exact: the same bits as scaling every term)
+      const float a0 = ca * sum[0] + cb * sum[1], a1 = cb * sum[0] + cc * sum[1];
       if (a0 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p)], a0);
       if (a1 != 0.f) atomicAdd(&out.d_pixxy[2 * size_t(p) + 1], a1);
       // d q / d(a, b, c) = (dx^2, 2 dx dy, dy^2)
-      const float dc[3] = {sum[2], 2.f * sum[3], sum[4]};
+      const float dc[3] = {-0.5f * sum[2], -sum[3], -0.5f * sum[4]};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (dc[c] != 0.f) atomicAdd(&out.d_conic[3 * size_t(p) + c], dc[c]);
