@@ -33,6 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "del-splat_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# VALU lane operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (a wave64 instruction takes 4 cycles; packed FP32 is not used)
+VALU_PEAK_TOPS = 39.32
+# operations per (pixel, list entry) evaluation of the adjoint, the survey's
+# model (~20 forward, 2.5x that backward)
+VALU_OPS_BWD = 50
 
 
 def parse():
@@ -154,22 +160,24 @@ def headline(args, torch, np, SG, scene):
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
         mark("project_bwd", False)
-        return i2p.numel()
+        return i2p.numel(), nwalk
 
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    K = 0
+    K, nwalk = 0, None
     for _ in range(args.steps):
-        K = step(True)
+        K, nwalk = step(True)
         torch.cuda.synchronize()
         for k in stages:
             acc[k] += ev[k][0].elapsed_time(ev[k][1])
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1000.0 / args.steps
     stage_ms = {k: acc[k] / args.steps for k in stages}
-    return s, m, shape, ms, stage_ms, K
+    # evaluations of the last frame: the sum of the per-pixel walk lengths
+    evals = int(nwalk.to(torch.int64).sum().item())
+    return s, m, shape, ms, stage_ms, K, evals
 
 
 def batched(args, torch, np, SG, scene):
@@ -345,7 +353,7 @@ def main():
     import del_splat_dlpack.SplatGauss.torch as SG
     from del_splat_dlpack import scene
 
-    s, m, shape, ms, stage_ms, K = headline(args, torch, np, SG, scene)
+    s, m, shape, ms, stage_ms, K, evals = headline(args, torch, np, SG, scene)
     N = args.num_gauss
     P = shape[0] * shape[1]
     # algorithmic bytes per launch (implementation independent, reference layouts)
@@ -393,6 +401,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        # the raster kernels issue VALU instructions back to back: the same
+        # kernel against the VALU peak (one op per lane per cycle, no packed
+        # FP32) in algorithmic terms, E evaluations x ops per evaluation
+        "valu": {"kernel": "raster_bwd", "evaluations": evals, "ops_per_eval": VALU_OPS_BWD,
+                 "achieved": round(evals * VALU_OPS_BWD / (stage_ms["raster_bwd"] * 1e-3) / 1e12, 2),
+                 "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                 "frac": round(evals * VALU_OPS_BWD / (stage_ms["raster_bwd"] * 1e-3) / 1e12
+                               / VALU_PEAK_TOPS, 4)},
     }
     if not args.no_batched:
         out["batched"] = batched(args, torch, np, SG, scene)
