@@ -146,16 +146,16 @@ def headline(args, torch, np, SG, scene):
         order = SG.tile_order(cost)
         hint[0] = order
         mark("raster_fwd", False)
+        # the L2 loss gradient 2 (img - target) is formed inside the adjoint;
+        # this stage zeroes its accumulators, so the raster_bwd events bracket
+        # the adjoint kernel alone
         mark("loss_grad", True)
-        d_img = 2.0 * (img - target)
-        # the adjoint's accumulators, zeroed here so the raster_bwd events
-        # bracket the adjoint kernel alone
         acc = torch.zeros(9 * splat2.shape[0], device=dev, dtype=torch.float32)
         mark("loss_grad", False)
         mark("raster_bwd", True)
-        dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast,
-                                                 d_img, None, args.threads_bwd, tile_order=order,
-                                                 grad_acc=acc)
+        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
+                                                    img, target, args.threads_bwd,
+                                                    tile_order=order, grad_acc=acc)
         mark("raster_bwd", False)
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)

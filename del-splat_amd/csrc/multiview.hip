@@ -588,6 +588,8 @@ static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const
   else if (ts == 16 && th == 64) launch_mv_bwd<16, 64>(P, s, out_rgb, d_img, target, d_alpha, out);
   else if (ts == 16 && th == 128) launch_mv_bwd<16, 128>(P, s, out_rgb, d_img, target, d_alpha, out);
   else if (ts == 16) launch_mv_bwd<16, 256>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else if (th == 1024) launch_mv_bwd<32, 1024>(P, s, out_rgb, d_img, target, d_alpha, out);
+  else if (th == 512) launch_mv_bwd<32, 512>(P, s, out_rgb, d_img, target, d_alpha, out);
   else launch_mv_bwd<32, 256>(P, s, out_rgb, d_img, target, d_alpha, out);
   const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, logit = flags & DSPLAT_OPACITY_IS_LOGIT;
   const float4* q4 = reinterpret_cast<const float4*>(quat);

@@ -29,13 +29,14 @@ gauss_raster_bwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw,
                         const uint32_t* __restrict__ idx2pnt,
                         const uint32_t* __restrict__ pix2nwalk,
                         const float* __restrict__ pix2tlast, const float* __restrict__ d_img,
+                        const float* __restrict__ pix2rgb, const float* __restrict__ target,
                         const float* __restrict__ d_alpha, float* __restrict__ d_pixxy,
                         float* __restrict__ d_conic, float* __restrict__ d_op,
                         float* __restrict__ d_rgb, const uint32_t* __restrict__ tile_order) {
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   const GradSink out{d_pixxy, d_conic, d_op, d_rgb};
   raster_tile_bwd<TS, NT>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile + 1], splat2,
-                          idx2pnt, pix2nwalk, pix2tlast, d_img, nullptr, nullptr, d_alpha, 0u,
+                          idx2pnt, pix2nwalk, pix2tlast, d_img, pix2rgb, target, d_alpha, 0u,
                           out);
 }
 
@@ -76,15 +77,17 @@ int launch_bwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
                const float4* splat2, const uint32_t* tile2idx, const uint32_t* idx2pnt,
                const uint32_t* nwalk, const float* tlast, const float* d_img,
                const float* d_alpha, float* d_pixxy, float* d_conic, float* d_op, float* d_rgb,
-               const uint32_t* order = nullptr) {
+               const uint32_t* order = nullptr, const float* rgb = nullptr,
+               const float* target = nullptr) {
   uint32_t tw, th;
   tile_grid(w, h, ts, &tw, &th);
   const uint32_t nt = tw * th;
   if (nt == 0) return DSPLAT_OK;
 #define DSPLAT_BWD(TS_, NT_)                                                            \
   gauss_raster_bwd_kernel<TS_, NT_><<<nt, NT_, 0, s>>>(w, h, tw, splat2, tile2idx, idx2pnt, \
-                                                       nwalk, tlast, d_img, d_alpha, d_pixxy, \
-                                                       d_conic, d_op, d_rgb, order)
+                                                       nwalk, tlast, d_img, rgb, target,      \
+                                                       d_alpha, d_pixxy, d_conic, d_op, d_rgb, \
+                                                       order)
   if (ts == 8) {
     DSPLAT_BWD(8, 64);
   } else if (ts == 16) {
@@ -193,4 +196,21 @@ extern "C" int dsplat_gauss_rasterize_bwd_ordered(
                     reinterpret_cast<const float4*>(pnt2splat2), tile2idx, idx2pnt, pix2nwalk,
                     pix2tlast, d_pix2rgb, d_pix2alpha, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
                     d_pnt2rgb, tile_order);
+}
+
+extern "C" int dsplat_gauss_rasterize_bwd_l2(
+    void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size, uint32_t threads,
+    const float* pnt2splat2, const uint32_t* tile2idx, const uint32_t* idx2pnt,
+    const uint32_t* tile_order, const uint32_t* pix2nwalk, const float* pix2tlast,
+    const float* pix2rgb, const float* target, float* d_pnt2pixxy, float* d_pnt2conic,
+    float* d_pnt2opacity, float* d_pnt2rgb) {
+  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tlast || !pix2rgb || !target ||
+      !d_pnt2pixxy || !d_pnt2conic || !d_pnt2opacity || !d_pnt2rgb)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd_l2: null buffer");
+  if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd_l2: splat2 must be 16-byte aligned");
+  return launch_bwd(as_stream(stream), img_w, img_h, tile_size, threads,
+                    reinterpret_cast<const float4*>(pnt2splat2), tile2idx, idx2pnt, pix2nwalk,
+                    pix2tlast, nullptr, nullptr, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
+                    d_pnt2rgb, tile_order, pix2rgb, target);
 }

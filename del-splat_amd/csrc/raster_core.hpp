@@ -182,8 +182,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     if (threadIdx.x < uint32_t(B) && it < num) {
       float4 q0, q1, q2, q3;
       ld.load(idx2pnt[end - 1 - it], q0, q1, q2, q3);
-      s0[threadIdx.x] = q0;
-      s1[threadIdx.x] = q1;
+      // the conic scaled for exp2 once per entry (every wave uses it)
+      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
+      s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
       s3[threadIdx.x] = q3;
       smask[threadIdx.x] = region_mask<TS, NT>(tx, ty, q2.y, q2.z, q2.w, q3.x);
@@ -209,7 +211,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       const float4 q3 = s3[j];
       const float4 q0 = s0[j];
       const float4 q1 = s1[j];
-      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (!((hits >> g) & 1u)) continue;  // wave-uniform
@@ -564,8 +566,9 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         if (lane < 8) acc[lane][j] = tot;
         if (lane == 63) acc[8][j] = vb[0];
       } else {
-        if (lane < 8) atomicAdd(&acc[lane][j], tot);
-        if (lane == 63) atomicAdd(&acc[8][j], vb[0]);
+        // one atomic with a per-lane slot: a lane-uniform address would be
+        // expanded by the compiler into a loop over the active lanes
+        if (lane < 8 || lane == 63) atomicAdd(&acc[lane < 8 ? lane : 8][j], lane < 8 ? tot : vb[0]);
       }
     }
     __syncthreads();

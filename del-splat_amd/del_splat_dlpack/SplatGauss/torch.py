@@ -193,6 +193,37 @@ def project_bwd(xyz, scale, quat, m16, img_shape, d_pixxy, d_conic, log_scale=Tr
     return d_xyz, d_scale, d_quat
 
 
+def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, img, target,
+                     threads=0, tile_order=None, grad_acc=None):
+    """rasterize_bwd for the loss sum (img - target)^2: dL/drgb = 2 (img -
+    target) is formed per pixel inside the adjoint kernel (img is the
+    forward's output, target [H,W,3]). Returns the same four 2D gradients."""
+    dev = splat2.device
+    n = splat2.shape[0]
+    w, h = img_shape
+    for t, name in ((img, "img"), (target, "target")):
+        check_f32(t, (h, w, 3), name)
+    if grad_acc is None:
+        acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
+    else:
+        if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
+                or not grad_acc.is_contiguous() or grad_acc.device != dev):
+            raise ValueError("grad_acc must be a contiguous float32 [9N] tensor on the "
+                             "splats' device")
+        acc = grad_acc.view(-1)
+    if tile_order is not None:
+        _check_schedule(tile2idx, tile_order)
+    d_pixxy = acc[:2 * n].view(n, 2)
+    d_conic = acc[2 * n:5 * n].view(n, 3)
+    d_op = acc[5 * n:6 * n]
+    d_rgb = acc[6 * n:].view(n, 3)
+    i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
+    call("dsplat_gauss_rasterize_bwd_l2", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
+         ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tlast), ptr(img.contiguous()),
+         ptr(target.contiguous()), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
+    return d_pixxy, d_conic, d_op, d_rgb
+
+
 def bin_gauss(img_shape, tile_size, pxd, aabb):
     return tile2idx_idx2pnt(img_shape, tile_size, pxd, aabb)
 

@@ -52,6 +52,7 @@ _SIGS = {
     "dsplat_gauss_rasterize_fwd_ordered": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_gauss_rasterize_bwd_ordered": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_tile_order": (_int, [_vp, _u32, _vp, _vp]),
+    "dsplat_gauss_rasterize_bwd_l2": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_gauss_project_bwd": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_circle_pnt2splat3_to_pnt2splat2": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _f32]),
     "dsplat_circle_tile2idx_idx2pnt_ws": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _sz, _vp]),

@@ -203,6 +203,19 @@ int dsplat_gauss_rasterize_bwd_ordered(void* stream, uint32_t img_w, uint32_t im
 int dsplat_tile_order(void* stream, uint32_t num_tile, const uint32_t* tile_cost,
                       uint32_t* tile_order);
 
+/* The adjoint of the L2 loss sum (pix2rgb - target)^2 over the image (pix2rgb
+ * the forward's output, target [H,W,3]) fused into the raster adjoint: the
+ * kernel forms dL/drgb = 2 (pix2rgb - target) per pixel, so no image-sized
+ * gradient is written or read. Otherwise as dsplat_gauss_rasterize_bwd_ordered
+ * without dL/dalpha (no counterpart in the reference, which has no adjoint). */
+int dsplat_gauss_rasterize_bwd_l2(void* stream, uint32_t img_w, uint32_t img_h,
+                                  uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
+                                  const uint32_t* tile2idx, const uint32_t* idx2pnt,
+                                  const uint32_t* tile_order, const uint32_t* pix2nwalk,
+                                  const float* pix2tlast, const float* pix2rgb,
+                                  const float* target, float* d_pnt2pixxy, float* d_pnt2conic,
+                                  float* d_pnt2opacity, float* d_pnt2rgb);
+
 /* Adjoint of the projection. ADDS into d_pnt2xyz [N,3], d_pnt2scale [N,3]
  * (w.r.t. sigma, or log sigma under DSPLAT_SCALE_IS_LOG), d_pnt2quat [N,4];
  * with DSPLAT_GRAD_OVERWRITE in flags it stores them instead (no zeroing

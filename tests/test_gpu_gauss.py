@@ -128,10 +128,9 @@ def test_binning_one_call_and_count_only(oracle):
 
 @pytest.mark.parametrize("n,shape", [(5000, (64, 64)), (20000, (64, 64)), (20000, (40, 30))])
 def test_binning_dense_tiles_and_equal_depths(oracle, n, shape):
-    """Every box covers the whole image, so each tile's list holds all n
-    Gaussians: n = 5000 sorts each tile in LDS, n = 20000 exceeds the LDS
-    segment capacity (8192) and streams; a third of the depths repeat three
-    values, so the equal-key path (index order within a depth) runs too."""
+    """Every box covers the whole image, so each tile's list holds (nearly)
+    all n Gaussians; a third of the depths repeat three values, so long runs
+    of equal depth keys must keep ascending index order."""
     import del_splat_dlpack.SplatGauss.torch as SG
     rng = np.random.default_rng(n + shape[0])
     boxes = np.tile(np.array([[-1.0, -1.0, shape[0] + 1.0, shape[1] + 1.0]], np.float32), (n, 1))
@@ -147,9 +146,8 @@ def test_binning_dense_tiles_and_equal_depths(oracle, n, shape):
 
 
 def test_binning_large_tile_grid(oracle):
-    """A 1024x1024 image at 8-pixel tiles (16384 tiles) is binned by the
-    general path (depth sort, pair fill, tile sort); same lists as the
-    oracle."""
+    """A 1024x1024 image at 8-pixel tiles: 16384 tiles, so the tile sort
+    runs on 14 key bits (two 7-bit passes); same lists as the oracle."""
     import del_splat_dlpack.SplatGauss.torch as SG
     shape = (1024, 1024)
     s, m = small_scene(20000, 8, shape)
@@ -364,6 +362,31 @@ def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
         err = max_rel(a.cpu().numpy(), b)
         print(f"ts={ts} threads={threads} alpha={with_alpha} {name}: max rel err {err:.2e}")
         assert err <= 2e-3, (name, err)
+
+
+@pytest.mark.parametrize("ts,threads", [(16, 0), (32, 512)])
+def test_raster_bwd_fused_l2(ts, threads):
+    """The fused-L2 adjoint (dL/drgb = 2 (img - target) formed in the kernel)
+    against the adjoint fed that gradient explicitly, with a tile schedule."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import scene
+    shape = (333, 201)
+    s, m = small_scene(20000, 21, shape)
+    xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, _t(m), shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
+    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, threads,
+                                                tile_cost=cost)
+    order = SG.tile_order(cost)
+    target = _t(scene.target_image(shape, 77))
+    ref = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast, 2.0 * (img - target), None,
+                           threads, tile_order=order)
+    got = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast, img, target, threads,
+                              tile_order=order)
+    for a, b in zip(got, ref):
+        # same terms, float atomics in another order
+        assert max_rel(a.cpu().numpy(), b.cpu().numpy()) <= 1e-5
 
 
 def test_project_bwd_add_and_overwrite():

@@ -61,13 +61,14 @@ def test_multiview_forward_matches_per_view(oracle, ts, threads):
     plan.close()
 
 
-def test_multiview_backward_matches_sum_of_per_view():
+@pytest.mark.parametrize("ts,threads", [(16, 0), (32, 512)])
+def test_multiview_backward_matches_sum_of_per_view(ts, threads):
     import del_splat_dlpack.SplatGauss.torch as SG
     from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
     s, views = _setup(n=8000, seed=41)
     n = s["xyz"].shape[0]
     base = {k: _t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb")}
-    plan = MultiViewPlan(n, views, tile_size=16)
+    plan = MultiViewPlan(n, views, tile_size=ts, threads=threads)
     targets = [_t(scene.target_image(shape, 50 + v)) for v, (_, shape) in enumerate(views)]
     target_flat = torch.cat([t.reshape(-1) for t in targets])
     out_rgb, _ = plan.forward(base["xyz"], base["log_scale"], base["quat"], base["opacity"],
