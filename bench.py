@@ -125,13 +125,15 @@ def headline(args, torch, np, SG, scene):
     # its own; results do not depend on the order
     tw, th = (shape[0] + ts - 1) // ts, (shape[1] + ts - 1) // ts
     hint = [None]
-    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for k in stages}
-    acc = {k: 0.0 for k in stages}
+    # one pair of events per stage and timed step, read after the last step:
+    # steps are not synchronised one by one, so the host queues the next
+    # frame while the GPU finishes the current one (as a training loop does)
+    evs = [{k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            for k in stages} for _ in range(args.steps)]
 
-    def step(timed):
+    def step(ev):
         def mark(name, first):
-            if timed:
+            if ev is not None:
                 ev[name][0 if first else 1].record()
         mark("project", True)
         pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
@@ -150,12 +152,12 @@ def headline(args, torch, np, SG, scene):
         # this stage zeroes its accumulators, so the raster_bwd events bracket
         # the adjoint kernel alone
         mark("loss_grad", True)
-        acc = torch.zeros(9 * splat2.shape[0], device=dev, dtype=torch.float32)
+        gacc = torch.zeros(9 * splat2.shape[0], device=dev, dtype=torch.float32)
         mark("loss_grad", False)
         mark("raster_bwd", True)
         dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
                                                     img, target, args.threads_bwd,
-                                                    tile_order=order, grad_acc=acc)
+                                                    tile_order=order, grad_acc=gacc)
         mark("raster_bwd", False)
         mark("project_bwd", True)
         SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
@@ -163,18 +165,16 @@ def headline(args, torch, np, SG, scene):
         return i2p.numel(), nwalk
 
     for _ in range(args.warmup):
-        step(False)
+        step(None)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     K, nwalk = 0, None
-    for _ in range(args.steps):
-        K, nwalk = step(True)
-        torch.cuda.synchronize()
-        for k in stages:
-            acc[k] += ev[k][0].elapsed_time(ev[k][1])
+    for i in range(args.steps):
+        K, nwalk = step(evs[i])
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1000.0 / args.steps
-    stage_ms = {k: acc[k] / args.steps for k in stages}
+    stage_ms = {k: sum(ev[k][0].elapsed_time(ev[k][1]) for ev in evs) / args.steps
+                for k in stages}
     # evaluations of the last frame: the sum of the per-pixel walk lengths
     evals = int(nwalk.to(torch.int64).sum().item())
     return s, m, shape, ms, stage_ms, K, evals
