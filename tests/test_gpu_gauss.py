@@ -447,3 +447,52 @@ def test_tile_schedule_orders_and_results():
                           tile_order=SG.tile_order(tc))
     for a, b in zip(g0, g1):
         assert max_rel(b.cpu().numpy(), a.cpu().numpy()) <= 1e-5
+
+
+def test_headline_frame_matches_oracle(oracle):
+    """The headline workload at full size (1M Gaussians, 1920x1080, 32x32
+    tiles, 512-thread forward and adjoint, walk-ordered tiles, fused L2
+    adjoint): projection and lists bit-exact, pixels within 1e-4 (termination
+    flips bounded), every gradient within 2e-3 of its largest magnitude, all
+    against the oracle's frame on the same inputs."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import scene
+    shape, ts, th = (1920, 1080), 32, 512
+    s = scene.gaussians(1_000_000, 2, radius=2.0)
+    m = scene.front_camera(shape)
+    target = scene.target_image(shape, 102)
+    # linear scales on both sides (the same float inputs for a bit-exact
+    # projection)
+    xyz, sc, q, op, rgb = (_t(s[k]) for k in ("xyz", "scale", "quat", "opacity", "rgb"))
+    m16 = _t(m)
+    pxd, conic, aabb, splat2 = SG.project_fwd(xyz, sc, q, op, rgb, m16, shape, log_scale=False)
+    t2i, i2p = SG.bin_gauss(shape, ts, pxd, aabb)
+    cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
+    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, tile_cost=cost)
+    order = SG.tile_order(cost)
+    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast, img,
+                                                _t(target), th, tile_order=order)
+    dxyz, dsc, dq = SG.project_bwd(xyz, sc, q, m16, shape, dpix, dcon, log_scale=False)
+    # the oracle's frame
+    o_pxd, o_conic, o_aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
+    np.testing.assert_array_equal(pxd.cpu().numpy(), o_pxd)
+    np.testing.assert_array_equal(conic.cpu().numpy(), o_conic)
+    np.testing.assert_array_equal(aabb.cpu().numpy(), o_aabb)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(o_pxd, o_aabb, shape, ts)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+        shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
+    flips = _check_pixels(img.cpu().numpy(), alpha.cpu().numpy(),
+                          nwalk.cpu().numpy().view(np.uint32), o_img, o_alpha, o_nwalk)
+    print(f"threshold flips: {flips}")
+    o_dpix, o_dcon, o_dop, o_drgb = oracle.gauss_rasterize_bwd(
+        shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p, o_nwalk,
+        o_tlast, (2.0 * (o_img - target)).astype(np.float32))
+    o_dxyz, o_dsc, o_dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape,
+                                                   o_dpix, o_dcon)
+    for name, g, o in (("xyz", dxyz, o_dxyz), ("scale", dsc, o_dsc), ("quat", dq, o_dq),
+                       ("opacity", dop, o_dop), ("rgb", drgb, o_drgb)):
+        err = max_rel(g.cpu().numpy().reshape(-1), np.asarray(o).reshape(-1))
+        print(f"{name}: max rel err {err:.2e}")
+        assert err <= 2e-3, (name, err)
