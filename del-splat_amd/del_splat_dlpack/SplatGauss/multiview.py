@@ -50,6 +50,10 @@ class MultiViewPlan:
         call("dsplat_mv_plan_create", self.num_pnt, self.max_view, self.max_pixels,
              self.tile_size, self.threads, self.capacity, ctypes.byref(self._plan))
         self.graph_kind = None
+        # where the last forward wrote alpha: a caller tensor, or None for the
+        # plan's own buffer (forwards given no out_alpha, training steps)
+        self._last_alpha = None
+        self._graph_alpha = None
         self.set_views(views)
 
     # ------------------------------------------------------------- views --
@@ -89,6 +93,7 @@ class MultiViewPlan:
         out_alpha = self.new_image(1) if out_alpha is None else out_alpha
         call("dsplat_mv_forward", self._plan, stream_ptr(self.device), ptr(xyz), ptr(scale),
              ptr(quat), ptr(opacity), ptr(rgb), flags, ptr(out_rgb), ptr(out_alpha))
+        self._last_alpha = out_alpha
         return out_rgb, out_alpha
 
     def backward(self, xyz, scale, quat, opacity, out_rgb=None, d_img=None, target=None,
@@ -108,6 +113,7 @@ class MultiViewPlan:
         call("dsplat_mv_train_step", self._plan, stream_ptr(self.device), ptr(params), ptr(grads),
              ptr(m1), ptr(m2), ptr(d_step), ptr(targets), ptr(out_rgb), lr, betas[0], betas[1],
              eps)
+        self._last_alpha = None
 
     # ------------------------------------------------------------ graphs --
     def capture_forward(self, xyz, scale, quat, opacity, rgb, out_rgb, out_alpha,
@@ -116,6 +122,7 @@ class MultiViewPlan:
              ptr(opacity), ptr(rgb), flags, ptr(out_rgb), ptr(out_alpha), None, None, None, None,
              None, None, 0.0, 0.0, 0.0, 0.0)
         self.graph_kind = "forward"
+        self._graph_alpha = out_alpha
 
     def capture_fwd_bwd(self, xyz, scale, quat, opacity, rgb, out_rgb, targets, grads,
                         flags=_native.DSPLAT_SCALE_IS_LOG):
@@ -125,6 +132,7 @@ class MultiViewPlan:
              ptr(opacity), ptr(rgb), flags, ptr(out_rgb), None, None, ptr(grads), None, None,
              None, ptr(targets), 0.0, 0.0, 0.0, 0.0)
         self.graph_kind = "fwd_bwd"
+        self._graph_alpha = None
 
     def capture_train_step(self, params, grads, m1, m2, d_step, targets, out_rgb, lr=1e-3,
                            betas=(0.9, 0.999), eps=1e-8):
@@ -132,9 +140,11 @@ class MultiViewPlan:
              ptr(out_rgb), None, ptr(params), ptr(grads), ptr(m1), ptr(m2), ptr(d_step),
              ptr(targets), lr, betas[0], betas[1], eps)
         self.graph_kind = "train"
+        self._graph_alpha = None
 
     def replay(self):
         call("dsplat_mv_graph_launch", self._plan, stream_ptr(self.device))
+        self._last_alpha = self._graph_alpha
 
     # ------------------------------------------------------------ state --
     def _outputs(self):
@@ -167,9 +177,14 @@ class MultiViewPlan:
                 self._copy(o[5], k, torch.int32))
 
     def state(self):
-        """(alpha, nwalk, tlast) of the last forward, concatenated per pixel."""
+        """(alpha, nwalk, tlast) of the last forward, concatenated per pixel
+        (alpha read from where that forward wrote it)."""
         o = self._outputs()
-        return (self._copy(o[1], self.total_pix, torch.float32),
+        if self._last_alpha is not None:
+            alpha = self._last_alpha.reshape(-1)[:self.total_pix].clone()
+        else:
+            alpha = self._copy(o[1], self.total_pix, torch.float32)
+        return (alpha,
                 self._copy(o[2], self.total_pix, torch.int32),
                 self._copy(o[3], self.total_pix, torch.float32))
 

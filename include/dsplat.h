@@ -343,8 +343,9 @@ int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
 int dsplat_mv_plan_info(const dsplat_mv_plan* plan, uint32_t* num_view, uint64_t* total_pix,
                         uint32_t* total_tiles, uint64_t* pix_offsets, uint32_t* tile_offsets);
 
-/* Device pointers of plan-owned state: alpha [pix], nwalk u32 [pix], tlast
- * [pix], tile2idx u32 [tiles+1] (global tile ids), idx2pnt u32 [capacity]
+/* Device pointers of plan-owned state: alpha [pix] (written only by a
+ * forward given a null out_alpha, and by the training step), nwalk u32 [pix],
+ * tlast [pix], tile2idx u32 [tiles+1] (global tile ids), idx2pnt u32 [capacity]
  * (item = view*N + Gaussian), counters u32 [5] = (K, K if it fit else 0, -, -,
  * overflow flag). Any argument may be null. rgb is reserved (null). */
 int dsplat_mv_outputs(const dsplat_mv_plan* plan, float** rgb, float** alpha, uint32_t** nwalk,

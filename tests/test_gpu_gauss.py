@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from _helpers import max_rel, small_scene, u32
+from _helpers import check_pixels as _check_pixels, max_rel, small_scene, u32
 
 pytestmark = pytest.mark.gpu
 
@@ -236,23 +236,6 @@ def test_exclusive_scan(n):
     if n:
         ref = np.concatenate([[0], np.cumsum(a, dtype=np.uint64)[:-1]]).astype(np.uint32)
         np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), ref)
-
-
-def _check_pixels(g_img, g_alpha, g_nwalk, o_img, o_alpha, o_nwalk, tol=1e-4):
-    """Pixels agree within tol; where the two sides' termination differs the
-    oracle's alpha must sit within 1e-5 of the 0.999 threshold."""
-    diff = np.abs(g_img - o_img).max(axis=2)
-    flip = g_nwalk != o_nwalk
-    bad = (diff > tol) & ~flip
-    assert not bad.any(), f"{bad.sum()} pixels off by up to {diff[bad].max()}"
-    if flip.any():
-        # a flip can only happen when the running alpha crosses 0.999 within
-        # rounding distance on one side
-        near = np.abs(np.maximum(g_alpha, o_alpha) - 0.999) < 2e-3
-        assert near[flip].all()
-        assert flip.mean() < 1e-3, flip.mean()
-    assert np.abs(g_alpha - o_alpha)[~flip].max(initial=0) <= tol
-    return int(flip.sum())
 
 
 @pytest.mark.parametrize("ts", [8, 16, 32])

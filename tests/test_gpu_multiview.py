@@ -58,6 +58,8 @@ def test_multiview_forward_matches_per_view(oracle, ts, threads):
         np.testing.assert_array_equal(imgs[v].cpu().numpy(), img.cpu().numpy())
         pa, pb = plan.pix_off[v], plan.pix_off[v + 1]
         np.testing.assert_array_equal(nwalk_all[pa:pb].cpu().numpy(), nwalk.reshape(-1).cpu().numpy())
+        np.testing.assert_array_equal(alpha_all[pa:pb].cpu().numpy(), alpha.reshape(-1).cpu().numpy())
+        np.testing.assert_array_equal(out_alpha[pa:pb].cpu().numpy(), alpha.reshape(-1).cpu().numpy())
     plan.close()
 
 
@@ -98,7 +100,7 @@ def test_graph_replay_forward():
     xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
     plan = MultiViewPlan(n, views, tile_size=16)
     eager, eager_a = plan.forward(xyz, ls, q, op, rgb)
-    eager = eager.clone()
+    eager, eager_a = eager.clone(), eager_a.clone()
     out_rgb = plan.new_image(3)
     out_a = plan.new_image(1)
     plan.capture_forward(xyz, ls, q, op, rgb, out_rgb, out_a)
@@ -107,6 +109,7 @@ def test_graph_replay_forward():
         plan.replay()
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out_rgb.cpu().numpy(), eager.cpu().numpy())
+        np.testing.assert_array_equal(plan.state()[0].cpu().numpy(), eager_a.cpu().numpy())
         # eager work between replays (a different frame into other buffers)
         plan.forward(xyz * 1.01, ls, q, op, rgb)
         torch.cuda.synchronize()
