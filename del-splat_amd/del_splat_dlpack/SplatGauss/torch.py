@@ -193,6 +193,86 @@ def project_bwd(xyz, scale, quat, m16, img_shape, d_pixxy, d_conic, log_scale=Tr
     return d_xyz, d_scale, d_quat
 
 
+def _cam_pos(cam_pos, device):
+    if torch.is_tensor(cam_pos):
+        c = cam_pos.to(device=device, dtype=torch.float32).reshape(-1).contiguous()
+    else:
+        c = torch.tensor([float(v) for v in cam_pos], device=device, dtype=torch.float32)
+    if c.numel() != 3:
+        raise ValueError("cam_pos must hold 3 values")
+    return c
+
+
+def _check_sh(xyz, sh, degree):
+    require_gpu(xyz, "xyz")
+    n = xyz.shape[0]
+    check_f32(xyz, (n, 3), "xyz")
+    if sh.dim() != 3 or sh.shape[0] != n or sh.shape[2] != 3 or sh.shape[1] not in (1, 4, 9, 16):
+        raise ValueError(f"sh must be [N, K, 3] with K in (1, 4, 9, 16); got {tuple(sh.shape)}")
+    check_f32(sh, tuple(sh.shape), "sh")
+    if not 0 <= degree <= 3 or (degree + 1) ** 2 > sh.shape[1]:
+        raise ValueError(f"degree {degree} needs {(degree + 1) ** 2} coefficients, sh has "
+                         f"{sh.shape[1]}")
+    if not (xyz.is_contiguous() and sh.is_contiguous()):
+        raise ValueError("xyz and sh must be contiguous")
+
+
+def sh_to_rgb_fwd(xyz, sh, cam_pos, degree):
+    """View-dependent colour [N,3] of spherical-harmonic coefficients sh
+    [N,K,3] seen from cam_pos (no counterpart in the reference)."""
+    _check_sh(xyz, sh, degree)
+    dev = xyz.device
+    rgb = torch.empty((xyz.shape[0], 3), device=dev, dtype=torch.float32)
+    call("dsplat_sh_to_rgb", stream_ptr(dev), xyz.shape[0], degree, sh.shape[1],
+         ptr(_cam_pos(cam_pos, dev)), ptr(xyz), ptr(sh), ptr(rgb))
+    return rgb
+
+
+def sh_to_rgb_bwd(xyz, sh, cam_pos, degree, d_rgb, d_sh=None, d_xyz=None):
+    """(d_sh [N,K,3], d_xyz [N,3]) of the colour's adjoint; added into d_sh
+    and d_xyz when both are given, stored in new tensors otherwise."""
+    _check_sh(xyz, sh, degree)
+    dev = xyz.device
+    n = xyz.shape[0]
+    check_f32(d_rgb, (n, 3), "d_rgb")
+    add = d_sh is not None and d_xyz is not None
+    if not add:
+        d_sh = torch.empty_like(sh)
+        d_xyz = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    check_f32(d_sh, tuple(sh.shape), "d_sh")
+    check_f32(d_xyz, (n, 3), "d_xyz")
+    if not (d_rgb.is_contiguous() and d_sh.is_contiguous() and d_xyz.is_contiguous()):
+        raise ValueError("d_rgb, d_sh and d_xyz must be contiguous")
+    call("dsplat_sh_to_rgb_bwd", stream_ptr(dev), n, degree, sh.shape[1],
+         ptr(_cam_pos(cam_pos, dev)), ptr(xyz), ptr(sh), ptr(d_rgb),
+         0 if add else _native.DSPLAT_GRAD_OVERWRITE, ptr(d_sh), ptr(d_xyz))
+    return d_sh, d_xyz
+
+
+class _ShColour(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, sh, cam, degree):
+        xyz, sh = xyz.contiguous(), sh.contiguous()
+        ctx.save_for_backward(xyz, sh, cam)
+        ctx.degree = degree
+        return sh_to_rgb_fwd(xyz, sh, cam, degree)
+
+    @staticmethod
+    def backward(ctx, d_rgb):
+        xyz, sh, cam = ctx.saved_tensors
+        d_sh, d_xyz = sh_to_rgb_bwd(xyz, sh, cam, ctx.degree, d_rgb.contiguous())
+        return d_xyz, d_sh, None, None
+
+
+def sh_to_rgb(xyz, sh, cam_pos, degree=3):
+    """Differentiable view-dependent colour: rgb [N,3] =
+    max(sum_k Y_k(normalize(xyz - cam_pos)) sh[:,k,:] + 1/2, 0), Y_k the real
+    spherical harmonics of 3D Gaussian splatting (k < (degree+1)^2). Feed the
+    result to `render` as its rgb; gradients reach sh and xyz."""
+    _check_sh(xyz, sh, degree)
+    return _ShColour.apply(xyz, sh, _cam_pos(cam_pos, xyz.device), int(degree))
+
+
 def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, img, target,
                      threads=0, tile_order=None, grad_acc=None):
     """rasterize_bwd for the loss sum (img - target)^2: dL/drgb = 2 (img -

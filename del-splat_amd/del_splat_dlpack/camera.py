@@ -64,3 +64,13 @@ def world2ndc(eye, target, img_shape, fov_y_deg=60.0, z_near=0.01, z_far=100.0,
 def to_col_major(m):
     """Row-major 4x4 -> column-major flat 16 float32 (m[i + 4 j] = M_ij)."""
     return np.ascontiguousarray(np.asarray(m, np.float64).T.reshape(16), dtype=np.float32)
+
+
+def camera_center(m):
+    """Eye point of a perspective world->ndc matrix (row-major 4x4, or the
+    column-major flat 16): the point where clip x, y and w all vanish."""
+    m = np.asarray(m, np.float64)
+    if m.shape == (16,):
+        m = m.reshape(4, 4).T
+    rows = m[[0, 1, 3]]
+    return np.linalg.solve(rows[:, :3], -rows[:, 3])

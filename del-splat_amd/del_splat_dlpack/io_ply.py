@@ -8,7 +8,9 @@ read_3d_gauss_splat  io_ply.rs:128-227 the 3DGS layout of exactly 62 float
                      activations: rgb = (f_dc + 0.5) * 0.5 (its sh_c0 is 0.5),
                      quaternion (r0,r1,r2,r3) -> (r1,r2,r3,r0) normalised,
                      scale = exp, opacity = sigmoid.
-Both expect the header shape the reference asserts (magic, format, one
+read_3d_gauss_splat_sh  the same file's raw SH coefficients [N,16,3] for the
+                     view-dependent colour (new).
+All expect the header shape the reference asserts (magic, format, one
 comment line, element vertex, the properties, end_header).
 """
 import numpy as np
@@ -55,8 +57,7 @@ def read_xyzrgb(path):
     return xyz, rgb
 
 
-def read_3d_gauss_splat(path):
-    """-> (xyz [N*3], rgb [N*3], sh [N*45], opacity [N], scale [N*3], quat [N*4])."""
+def _read_gauss_records(path):
     with open(path, "rb") as f:
         num, props, _ = _read_header(f, 62)
         for p in props:
@@ -65,7 +66,12 @@ def read_3d_gauss_splat(path):
         buf = f.read()
     if len(buf) != 62 * num * 4:
         raise ValueError("payload size mismatch")
-    a = np.frombuffer(buf, dtype="<f4").reshape(num, 62)
+    return np.frombuffer(buf, dtype="<f4").reshape(num, 62)
+
+
+def read_3d_gauss_splat(path):
+    """-> (xyz [N*3], rgb [N*3], sh [N*45], opacity [N], scale [N*3], quat [N*4])."""
+    a = _read_gauss_records(path)
     sh_c0 = np.float32(0.5)
     xyz = np.ascontiguousarray(a[:, 0:3])
     rgb = (a[:, 6:9] + np.float32(0.5)) * sh_c0
@@ -78,3 +84,18 @@ def read_3d_gauss_splat(path):
     q = (q / np.sqrt((q * q).sum(1, keepdims=True))).astype(np.float32)
     return (xyz.reshape(-1), rgb.astype(np.float32).reshape(-1), sh.reshape(-1), op,
             scale.reshape(-1), q.reshape(-1))
+
+
+def read_3d_gauss_splat_sh(path):
+    """Raw spherical-harmonic coefficients of the same layout as float32
+    [N, 16, 3] (coefficient-major, channel inner; the layout of the kernels'
+    SH colour): coefficient 0 is f_dc, coefficients 1..15 of channel c are
+    f_rest[15 c .. 15 c + 14] (the file stores the higher orders channel by
+    channel). No counterpart in the reference, whose reader folds f_dc into
+    its rgb and returns f_rest flat (io_ply.rs:196-204)."""
+    a = _read_gauss_records(path)
+    n = a.shape[0]
+    sh = np.empty((n, 16, 3), np.float32)
+    sh[:, 0, :] = a[:, 6:9]
+    sh[:, 1:, :] = a[:, 9:54].reshape(n, 3, 15).transpose(0, 2, 1)
+    return sh

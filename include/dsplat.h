@@ -227,6 +227,30 @@ int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const float* pnt2xy
                              uint32_t flags, const float* d_pnt2pixxy, const float* d_pnt2conic,
                              float* d_pnt2xyz, float* d_pnt2scale, float* d_pnt2quat);
 
+/* View-dependent colour from spherical harmonics (no counterpart in the
+ * reference: io_ply.rs:196 reads the 45 higher-order coefficients of the 3DGS
+ * PLY layout and the renderer never evaluates them).
+ * pnt2rgb[i][c] = max(sum_k Y_k(d_i) pnt2sh[i][k][c] + 1/2, 0), with
+ * d_i = normalize(pnt2xyz[i] - cam_pos) and Y_k the real spherical harmonics
+ * of 3D Gaussian splatting, k = l^2 + l + m < (degree+1)^2 (degree <= 3).
+ * pnt2sh is [N][num_coef][3], num_coef in {1, 4, 9, 16} with
+ * (degree+1)^2 <= num_coef, 16-byte aligned; cam_pos is a device float[3]
+ * (read by the kernel, so a captured graph follows later changes to it).
+ * To render with it, pass pnt2rgb as the colour of the projection. */
+int dsplat_sh_to_rgb(void* stream, uint32_t num_pnt, uint32_t degree, uint32_t num_coef,
+                     const float* cam_pos, const float* pnt2xyz, const float* pnt2sh,
+                     float* pnt2rgb);
+
+/* Adjoint of dsplat_sh_to_rgb: d_pnt2sh [N][num_coef][3] = Y_k g_c and
+ * d_pnt2xyz [N,3] = (I - d d^T)/|xyz - cam| * sum_{k,c} sh[k][c] g_c grad Y_k,
+ * with g = d_pnt2rgb where the colour was not clamped and 0 where it was.
+ * ADDS into both, or stores them under DSPLAT_GRAD_OVERWRITE. Coefficients
+ * beyond (degree+1)^2 get a zero gradient. */
+int dsplat_sh_to_rgb_bwd(void* stream, uint32_t num_pnt, uint32_t degree, uint32_t num_coef,
+                         const float* cam_pos, const float* pnt2xyz, const float* pnt2sh,
+                         const float* d_pnt2rgb, uint32_t flags, float* d_pnt2sh,
+                         float* d_pnt2xyz);
+
 /* ======================================================================
  * Circle path (del-splat-cudarc/src/splat_circle.rs) and point z-buffers
  * (del-splat-dlpack/src/splat_point.rs)

@@ -27,6 +27,7 @@
 //   transfer adjoint    del-splat-cpu/src/splat_gaussian2.rs:201-266 (commented
 //                       prototype; the full adjoint below is new)
 //   3DGS PLY reader     del-splat-cpu/src/io_ply.rs:128-227
+//   SH colour           new (coefficients read at io_ply.rs:196, unused there)
 //
 // The math helpers of the un-vendored del-geo crates (transform_homogeneous,
 // jacobian_transform, transform_ndc2pix, projected_spd_mat3,
@@ -693,6 +694,181 @@ void render_pix_sort_depth(uint32_t n, const float* pxz, const float* rgb, uint3
   }
 }
 
+// ---------------------------------------------------------------------------
+// View-dependent colour from spherical harmonics (new; the reference reads
+// the 45 higher-order coefficients, io_ply.rs:196, but never evaluates them).
+//
+// Basis: real spherical harmonics of degree l <= 3, coefficient k = l^2+l+m,
+//   Y_lm = sqrt(2) Re Y_l^|m| (m > 0), sqrt(2) Im Y_l^|m| (m < 0), Y_l^0,
+// of the complex harmonics Y_l^m with the Condon-Shortley phase (the basis
+// of 3D Gaussian splatting's colour model). Written as polynomials of the
+// unit direction d = (x, y, z) from the camera centre to the mean:
+//   k0  1/(2 sqrt(pi))
+//   k1..3   c1 (-y, z, -x),                       c1 = sqrt(3/(4 pi))
+//   k4..8   c2a xy, -c2a yz, c2b (2z^2-x^2-y^2), -c2a xz, c2c (x^2-y^2)
+//           c2a = sqrt(15/(4 pi)), c2b = sqrt(5/(16 pi)), c2c = sqrt(15/(16 pi))
+//   k9..15  -c3a y(3x^2-y^2), c3b xyz, -c3c y(4z^2-x^2-y^2),
+//           c3d z(2z^2-3x^2-3y^2), -c3c x(4z^2-x^2-y^2), c3e z(x^2-y^2),
+//           -c3a x(x^2-3y^2)
+//           c3a = sqrt(35/(32 pi)), c3b = sqrt(105/(4 pi)), c3c = sqrt(21/(32 pi)),
+//           c3d = sqrt(7/(16 pi)), c3e = sqrt(105/(16 pi))
+// Colour channel c: max(sum_k Y_k(d) sh[k][c] + 1/2, 0), sh laid out
+// [N][K][3] (coefficient-major, channel inner), K >= (degree+1)^2.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ShConst {
+  static T pi() { return T(3.14159265358979323846); }
+  static T k0() { return T(std::sqrt(1.0 / (4.0 * 3.14159265358979323846))); }
+  static T c1() { return T(std::sqrt(3.0 / (4.0 * 3.14159265358979323846))); }
+  static T c2a() { return T(std::sqrt(15.0 / (4.0 * 3.14159265358979323846))); }
+  static T c2b() { return T(std::sqrt(5.0 / (16.0 * 3.14159265358979323846))); }
+  static T c2c() { return T(std::sqrt(15.0 / (16.0 * 3.14159265358979323846))); }
+  static T c3a() { return T(std::sqrt(35.0 / (32.0 * 3.14159265358979323846))); }
+  static T c3b() { return T(std::sqrt(105.0 / (4.0 * 3.14159265358979323846))); }
+  static T c3c() { return T(std::sqrt(21.0 / (32.0 * 3.14159265358979323846))); }
+  static T c3d() { return T(std::sqrt(7.0 / (16.0 * 3.14159265358979323846))); }
+  static T c3e() { return T(std::sqrt(105.0 / (16.0 * 3.14159265358979323846))); }
+};
+
+// b[k] for k < (deg+1)^2.
+template <typename T>
+inline void sh_basis(T x, T y, T z, uint32_t deg, T* b) {
+  using C = ShConst<T>;
+  b[0] = C::k0();
+  if (deg < 1) return;
+  b[1] = -C::c1() * y;
+  b[2] = C::c1() * z;
+  b[3] = -C::c1() * x;
+  if (deg < 2) return;
+  const T xx = x * x, yy = y * y, zz = z * z;
+  b[4] = C::c2a() * (x * y);
+  b[5] = -C::c2a() * (y * z);
+  b[6] = C::c2b() * (T(2) * zz - xx - yy);
+  b[7] = -C::c2a() * (x * z);
+  b[8] = C::c2c() * (xx - yy);
+  if (deg < 3) return;
+  b[9] = -C::c3a() * (y * (T(3) * xx - yy));
+  b[10] = C::c3b() * (x * y * z);
+  b[11] = -C::c3c() * (y * (T(4) * zz - xx - yy));
+  b[12] = C::c3d() * (z * (T(2) * zz - T(3) * xx - T(3) * yy));
+  b[13] = -C::c3c() * (x * (T(4) * zz - xx - yy));
+  b[14] = C::c3e() * (z * (xx - yy));
+  b[15] = -C::c3a() * (x * (xx - T(3) * yy));
+}
+
+// g[3k + i] = d b[k] / d d_i of the polynomials above (taken in R^3; the
+// caller keeps only the part tangent to the sphere).
+template <typename T>
+inline void sh_basis_grad(T x, T y, T z, uint32_t deg, T* g) {
+  using C = ShConst<T>;
+  for (uint32_t i = 0; i < 48; ++i) g[i] = T(0);
+  if (deg < 1) return;
+  g[3 * 1 + 1] = -C::c1();
+  g[3 * 2 + 2] = C::c1();
+  g[3 * 3 + 0] = -C::c1();
+  if (deg < 2) return;
+  const T xx = x * x, yy = y * y, zz = z * z;
+  g[3 * 4 + 0] = C::c2a() * y;          g[3 * 4 + 1] = C::c2a() * x;
+  g[3 * 5 + 1] = -C::c2a() * z;         g[3 * 5 + 2] = -C::c2a() * y;
+  g[3 * 6 + 0] = -T(2) * C::c2b() * x;  g[3 * 6 + 1] = -T(2) * C::c2b() * y;
+  g[3 * 6 + 2] = T(4) * C::c2b() * z;
+  g[3 * 7 + 0] = -C::c2a() * z;         g[3 * 7 + 2] = -C::c2a() * x;
+  g[3 * 8 + 0] = T(2) * C::c2c() * x;   g[3 * 8 + 1] = -T(2) * C::c2c() * y;
+  if (deg < 3) return;
+  g[3 * 9 + 0] = -C::c3a() * (T(6) * x * y);
+  g[3 * 9 + 1] = -C::c3a() * (T(3) * xx - T(3) * yy);
+  g[3 * 10 + 0] = C::c3b() * (y * z);
+  g[3 * 10 + 1] = C::c3b() * (x * z);
+  g[3 * 10 + 2] = C::c3b() * (x * y);
+  g[3 * 11 + 0] = -C::c3c() * (-T(2) * x * y);
+  g[3 * 11 + 1] = -C::c3c() * (T(4) * zz - xx - T(3) * yy);
+  g[3 * 11 + 2] = -C::c3c() * (T(8) * y * z);
+  g[3 * 12 + 0] = C::c3d() * (-T(6) * x * z);
+  g[3 * 12 + 1] = C::c3d() * (-T(6) * y * z);
+  g[3 * 12 + 2] = C::c3d() * (T(6) * zz - T(3) * xx - T(3) * yy);
+  g[3 * 13 + 0] = -C::c3c() * (T(4) * zz - T(3) * xx - yy);
+  g[3 * 13 + 1] = -C::c3c() * (-T(2) * x * y);
+  g[3 * 13 + 2] = -C::c3c() * (T(8) * x * z);
+  g[3 * 14 + 0] = C::c3e() * (T(2) * x * z);
+  g[3 * 14 + 1] = C::c3e() * (-T(2) * y * z);
+  g[3 * 14 + 2] = C::c3e() * (xx - yy);
+  g[3 * 15 + 0] = -C::c3a() * (T(3) * xx - T(3) * yy);
+  g[3 * 15 + 1] = -C::c3a() * (-T(6) * x * y);
+}
+
+// Unit direction from the camera centre to the mean, and 1/|xyz - cam|.
+template <typename T>
+inline void sh_direction(const T* cam, const T* p, T* d, T* inv_len) {
+  const T vx = p[0] - cam[0], vy = p[1] - cam[1], vz = p[2] - cam[2];
+  const T inv = T(1) / std::sqrt(vx * vx + vy * vy + vz * vz);
+  d[0] = vx * inv;
+  d[1] = vy * inv;
+  d[2] = vz * inv;
+  *inv_len = inv;
+}
+
+template <typename T>
+void sh_to_rgb(uint32_t n, uint32_t deg, uint32_t K, const T* cam, const T* xyz, const T* sh,
+               T* rgb) {
+  const uint32_t nk = (deg + 1) * (deg + 1);
+  for (uint32_t i = 0; i < n; ++i) {
+    T d[3], inv, b[16];
+    sh_direction(cam, xyz + 3 * i, d, &inv);
+    sh_basis(d[0], d[1], d[2], deg, b);
+    const T* c = sh + size_t(i) * K * 3;
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+      T acc = T(0);
+      for (uint32_t k = 0; k < nk; ++k) acc += b[k] * c[3 * k + ch];
+      acc += T(0.5);
+      rgb[3 * i + ch] = acc > T(0) ? acc : T(0);
+    }
+  }
+}
+
+// Adjoint: d_sh[i][k][c] = Y_k g_c and d_xyz[i] = (I - d d^T) / |v| *
+// sum_{k,c} sh[k][c] g_c grad Y_k, with g_c = d_rgb[c] where the colour was
+// not clamped (sum + 1/2 >= 0) and 0 where it was. Stored (overwrite) or
+// added. Coefficients k >= (deg+1)^2 get no gradient (left, or stored 0).
+template <typename T>
+void sh_to_rgb_bwd(uint32_t n, uint32_t deg, uint32_t K, const T* cam, const T* xyz,
+                   const T* sh, const T* d_rgb, T* d_sh, T* d_xyz, bool overwrite) {
+  const uint32_t nk = (deg + 1) * (deg + 1);
+  for (uint32_t i = 0; i < n; ++i) {
+    T d[3], inv, b[16], gb[48];
+    sh_direction(cam, xyz + 3 * i, d, &inv);
+    sh_basis(d[0], d[1], d[2], deg, b);
+    sh_basis_grad(d[0], d[1], d[2], deg, gb);
+    const T* c = sh + size_t(i) * K * 3;
+    T g[3];
+    for (uint32_t ch = 0; ch < 3; ++ch) {
+      T acc = T(0);
+      for (uint32_t k = 0; k < nk; ++k) acc += b[k] * c[3 * k + ch];
+      acc += T(0.5);
+      g[ch] = acc < T(0) ? T(0) : d_rgb[3 * i + ch];
+    }
+    T* ds = d_sh + size_t(i) * K * 3;
+    for (uint32_t k = 0; k < K; ++k)
+      for (uint32_t ch = 0; ch < 3; ++ch) {
+        const T v = k < nk ? b[k] * g[ch] : T(0);
+        if (overwrite) ds[3 * k + ch] = v;
+        else ds[3 * k + ch] += v;
+      }
+    T dd[3] = {T(0), T(0), T(0)};
+    for (uint32_t k = 1; k < nk; ++k) {
+      const T w = c[3 * k] * g[0] + c[3 * k + 1] * g[1] + c[3 * k + 2] * g[2];
+      dd[0] += w * gb[3 * k];
+      dd[1] += w * gb[3 * k + 1];
+      dd[2] += w * gb[3 * k + 2];
+    }
+    const T r = d[0] * dd[0] + d[1] * dd[1] + d[2] * dd[2];
+    for (uint32_t a = 0; a < 3; ++a) {
+      const T v = (dd[a] - d[a] * r) * inv;
+      if (overwrite) d_xyz[3 * i + a] = v;
+      else d_xyz[3 * i + a] += v;
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -867,6 +1043,33 @@ void orc_adam(uint32_t n, float* param, const float* grad, float* m1, float* m2,
     const float mh = m1[i] / bc1, vh = m2[i] / bc2;
     param[i] -= lr * mh / (std::sqrt(vh) + eps);
   }
+}
+
+void orc_sh_basis_f64(double x, double y, double z, uint32_t deg, double* b, double* grad) {
+  sh_basis(x, y, z, deg, b);
+  if (grad) sh_basis_grad(x, y, z, deg, grad);
+}
+
+void orc_sh_to_rgb_f32(uint32_t n, uint32_t deg, uint32_t K, const float* cam, const float* xyz,
+                       const float* sh, float* rgb) {
+  sh_to_rgb(n, deg, K, cam, xyz, sh, rgb);
+}
+
+void orc_sh_to_rgb_f64(uint32_t n, uint32_t deg, uint32_t K, const double* cam,
+                       const double* xyz, const double* sh, double* rgb) {
+  sh_to_rgb(n, deg, K, cam, xyz, sh, rgb);
+}
+
+void orc_sh_to_rgb_bwd_f32(uint32_t n, uint32_t deg, uint32_t K, const float* cam,
+                           const float* xyz, const float* sh, const float* d_rgb, float* d_sh,
+                           float* d_xyz, int overwrite) {
+  sh_to_rgb_bwd(n, deg, K, cam, xyz, sh, d_rgb, d_sh, d_xyz, overwrite != 0);
+}
+
+void orc_sh_to_rgb_bwd_f64(uint32_t n, uint32_t deg, uint32_t K, const double* cam,
+                           const double* xyz, const double* sh, const double* d_rgb,
+                           double* d_sh, double* d_xyz, int overwrite) {
+  sh_to_rgb_bwd(n, deg, K, cam, xyz, sh, d_rgb, d_sh, d_xyz, overwrite != 0);
 }
 
 }  // extern "C"

@@ -90,6 +90,15 @@ def lib():
         L.orc_adam.restype = None
         L.orc_adam.argtypes = [_u32, _f32p, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float,
                                ctypes.c_float, ctypes.c_float, _u32]
+        L.orc_sh_basis_f64.restype = None
+        L.orc_sh_basis_f64.argtypes = [ctypes.c_double] * 3 + [_u32, _f64p, _vp]
+        for sfx, fp in (("f32", _f32p), ("f64", _f64p)):
+            f = getattr(L, "orc_sh_to_rgb_" + sfx)
+            f.restype = None
+            f.argtypes = [_u32, _u32, _u32, fp, fp, fp, fp]
+            f = getattr(L, "orc_sh_to_rgb_bwd_" + sfx)
+            f.restype = None
+            f.argtypes = [_u32, _u32, _u32, fp, fp, fp, fp, fp, fp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -307,3 +316,39 @@ def dw_transfer(alphas, i, j):
 def adam(param, grad, m1, m2, lr, b1, b2, eps, step):
     """In place on param/m1/m2 (float32 arrays)."""
     lib().orc_adam(param.size, param, _c(grad, np.float32), m1, m2, lr, b1, b2, eps, step)
+
+
+# ------------------------------------------------------------- SH colour
+def sh_basis(d, degree=3, with_grad=False):
+    """Real SH basis values (16) and optionally their gradients (16x3) at
+    the unit direction d, float64."""
+    b = np.zeros(16, np.float64)
+    g = np.zeros(48, np.float64) if with_grad else None
+    lib().orc_sh_basis_f64(float(d[0]), float(d[1]), float(d[2]), degree, b, _ptr(g))
+    return (b, g.reshape(16, 3)) if with_grad else b
+
+
+def sh_to_rgb(xyz, sh, cam, degree, dtype=np.float32):
+    """rgb [N,3] = max(sum_k Y_k(normalize(xyz - cam)) sh[:,k,:] + 1/2, 0);
+    sh is [N, K, 3]."""
+    sh = _c(sh, dtype)
+    n, K = sh.shape[0], sh.shape[1]
+    xyz, cam = _c(xyz, dtype).reshape(-1), _c(cam, dtype).reshape(-1)
+    rgb = np.empty(3 * n, dtype)
+    f = lib().orc_sh_to_rgb_f32 if dtype == np.float32 else lib().orc_sh_to_rgb_f64
+    f(n, degree, K, cam, xyz, sh.reshape(-1), rgb)
+    return rgb.reshape(n, 3)
+
+
+def sh_to_rgb_bwd(xyz, sh, cam, degree, d_rgb, dtype=np.float32, d_sh=None, d_xyz=None):
+    """(d_sh [N,K,3], d_xyz [N,3]); added into d_sh / d_xyz when given,
+    stored otherwise."""
+    sh = _c(sh, dtype)
+    n, K = sh.shape[0], sh.shape[1]
+    overwrite = d_sh is None
+    d_sh = np.zeros(n * K * 3, dtype) if d_sh is None else _c(d_sh, dtype).reshape(-1).copy()
+    d_xyz = np.zeros(3 * n, dtype) if d_xyz is None else _c(d_xyz, dtype).reshape(-1).copy()
+    f = lib().orc_sh_to_rgb_bwd_f32 if dtype == np.float32 else lib().orc_sh_to_rgb_bwd_f64
+    f(n, degree, K, _c(cam, dtype).reshape(-1), _c(xyz, dtype).reshape(-1), sh.reshape(-1),
+      _c(d_rgb, dtype).reshape(-1), d_sh, d_xyz, int(overwrite))
+    return d_sh.reshape(n, K, 3), d_xyz.reshape(n, 3)
