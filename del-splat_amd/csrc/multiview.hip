@@ -18,6 +18,12 @@
 // global tiles from an atomic counter until all tiles of all views are done,
 // so views of uneven resolution share one launch without tail gaps between
 // per-view grids.
+//
+// Colour is either one rgb per Gaussian or, after dsplat_mv_set_sh,
+// spherical-harmonic coefficients seen from each view's camera centre: a
+// per-Gaussian kernel stages the coefficients once and writes one colour per
+// item, the raster adjoint then leaves dL/drgb per item, and a per-Gaussian
+// kernel folds the views back into dL/dsh and the direction part of dL/dxyz.
 #include <stdlib.h>
 #include <string.h>
 
@@ -26,6 +32,7 @@
 #include "primitives.hpp"
 #include "project_math.hpp"
 #include "raster_core.hpp"
+#include "sh_core.hpp"
 
 namespace dsplat {
 
@@ -34,6 +41,8 @@ struct ViewDev {
   uint32_t w, h, tw, th;
   uint32_t tile_off, pad0;
   uint64_t pix_off;
+  float cam[3];  // camera centre (world), for the SH colour
+  float pad1;
 };
 
 constexpr uint32_t kMaxViews = 64;
@@ -50,7 +59,7 @@ __device__ inline uint32_t view_of_tile(const ViewDev* __restrict__ views, uint3
 __device__ inline float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
 
 // ---------------------------------------------------------- projection ---
-template <bool kLogScale, bool kLogitOp>
+template <bool kLogScale, bool kLogitOp, bool kRgbItem>
 __global__ void __launch_bounds__(256)
 mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
                   const float* __restrict__ xyz, const float* __restrict__ scale,
@@ -74,14 +83,102 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
   float op = opacity[i];
   if (kLogitOp) op = sigmoidf_(op);
   const size_t item = size_t(v) * n + i;
+  const float* col = rgb + 3 * (kRgbItem ? item : size_t(i));
   float4* r = rec + 4 * item;
   r[0] = make_float4(p.pix[0], p.pix[1], p.conic[0], p.conic[1]);
-  r[1] = make_float4(p.conic[2], op, rgb[3 * i], rgb[3 * i + 1]);
-  r[2] = make_float4(rgb[3 * i + 2], p.aabb[0], p.aabb[1], p.aabb[2]);
+  r[1] = make_float4(p.conic[2], op, col[0], col[1]);
+  r[2] = make_float4(col[2], p.aabb[0], p.aabb[1], p.aabb[2]);
   r[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
   const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
   cnt[item] = uint32_t(tr.count());
   dkey[item] = depth_key(p.ndcz);
+}
+
+// ----------------------------------------------------------- SH colour ---
+// One lane per Gaussian: its coefficient record is staged once, then one
+// colour per view goes to the item's slot (rgb_item [items, 3]).
+template <int K, int DEG>
+__global__ void __launch_bounds__(256)
+mv_sh_kernel(uint32_t n, uint32_t nview, const ViewDev* __restrict__ views,
+             const float* __restrict__ xyz, const float* __restrict__ coef,
+             float* __restrict__ rgb_item) {
+  constexpr int R = 3 * K, RP = sh::record_stride(R), NK = (DEG + 1) * (DEG + 1);
+  __shared__ __align__(16) float lds[4][64 * RP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t g0 = blockIdx.x * 256u + 64u * uint32_t(wave);
+  if (g0 >= n) return;  // wave-uniform
+  const uint32_t cnt = min(64u, n - g0);
+  sh::stage_in<R, RP>(coef, g0, cnt, lds[wave], lane);
+  sh::wave_lds_sync();
+  const uint32_t i = g0 + uint32_t(lane);
+  if (i >= n) return;
+  float c[R];
+  sh::load_record<R, RP>(lds[wave], lane, c);
+  const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+  for (uint32_t v = 0; v < nview; ++v) {
+    float d[3], inv, b[16], raw[3];
+    sh::direction(views[v].cam, px, py, pz, d, &inv);
+    sh::sh_basis<DEG>(d[0], d[1], d[2], b);
+    sh::colour<NK>(c, b, raw);
+    float* o = rgb_item + 3 * (size_t(v) * n + i);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) o[ch] = raw[ch] > 0.f ? raw[ch] : 0.f;
+  }
+}
+
+// The adjoint of mv_sh_kernel summed over views: d_sh (stored) and the
+// direction part of d_xyz (added to what the projection adjoint stored).
+template <int K, int DEG>
+__global__ void __launch_bounds__(256)
+mv_sh_bwd_kernel(uint32_t n, uint32_t nview, const ViewDev* __restrict__ views,
+                 const float* __restrict__ xyz, const float* __restrict__ coef,
+                 const float* __restrict__ d_rgb_item, float* __restrict__ d_coef,
+                 float* __restrict__ d_xyz) {
+  constexpr int R = 3 * K, RP = sh::record_stride(R), NK = (DEG + 1) * (DEG + 1);
+  __shared__ __align__(16) float lds[4][64 * RP];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t g0 = blockIdx.x * 256u + 64u * uint32_t(wave);
+  if (g0 >= n) return;  // wave-uniform
+  const uint32_t cnt = min(64u, n - g0);
+  float* L = lds[wave];
+  sh::stage_in<R, RP>(coef, g0, cnt, L, lane);
+  sh::wave_lds_sync();
+  const uint32_t i = g0 + uint32_t(lane);
+  float c[R];
+  sh::load_record<R, RP>(L, lane, c);
+  float ds[RP];
+#pragma unroll
+  for (int k = 0; k < RP; ++k) ds[k] = 0.f;
+  if (i < n) {
+    const float px = xyz[3 * i], py = xyz[3 * i + 1], pz = xyz[3 * i + 2];
+    float ax[3] = {0.f, 0.f, 0.f};
+    for (uint32_t v = 0; v < nview; ++v) {
+      const float* gi = d_rgb_item + 3 * (size_t(v) * n + i);
+      const float g0c = gi[0], g1c = gi[1], g2c = gi[2];
+      if (g0c == 0.f && g1c == 0.f && g2c == 0.f) continue;  // not seen in this view
+      float d[3], inv, b[16], raw[3];
+      sh::direction(views[v].cam, px, py, pz, d, &inv);
+      sh::sh_basis<DEG>(d[0], d[1], d[2], b);
+      sh::colour<NK>(c, b, raw);
+      const float gc[3] = {raw[0] < 0.f ? 0.f : g0c, raw[1] < 0.f ? 0.f : g1c,
+                           raw[2] < 0.f ? 0.f : g2c};
+#pragma unroll
+      for (int k = 0; k < NK; ++k)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) ds[3 * k + ch] += b[k] * gc[ch];
+      sh::direction_adjoint<DEG>(c, gc, d, inv, ax);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d_xyz[3 * i + a] += ax[a];
+  }
+  // every lane has its record in registers; write d_sh back through LDS
+  sh::wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < (R + 3) / 4; ++q)
+    *reinterpret_cast<float4*>(L + lane * RP + 4 * q) =
+        make_float4(ds[4 * q], ds[4 * q + 1], ds[4 * q + 2], ds[4 * q + 3]);
+  sh::wave_lds_sync();
+  sh::stage_out<R, RP, false>(d_coef, g0, cnt, L, lane);
 }
 
 // ------------------------------------------------------------- binning ---
@@ -327,16 +424,54 @@ struct dsplat_mv_plan {
   // it (the schedule of the next adjoint and forward; raster order after
   // set_views)
   uint32_t *tile_cost = nullptr, *tile_order = nullptr;
+  // SH colour (dsplat_mv_set_sh): coefficients per channel (0: plain rgb),
+  // degree, per-item colours and their gradients, the coefficients the last
+  // forward read, and whether every view has a camera centre
+  uint32_t sh_k = 0, sh_deg = 0, fwd_sh_k = 0, fwd_sh_deg = 0;
+  float *rgb_item = nullptr, *d_rgb_item = nullptr;
+  const float* fwd_sh = nullptr;
+  void* sh_mem = nullptr;
+  bool cams_ok = false;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap_stream = nullptr;
 };
 
 static size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Camera centre of a perspective world->ndc matrix (column-major, M_ij =
+// m[i + 4 j]): the point where clip x, y and w all vanish, i.e. A c = -b over
+// rows 0, 1, 3 (Cramer's rule in double). False if those rows are singular.
+static bool camera_centre(const float* m, float* c) {
+  double a[3][3], b[3];
+  const int rows[3] = {0, 1, 3};
+  for (int r = 0; r < 3; ++r) {
+    for (int j = 0; j < 3; ++j) a[r][j] = m[rows[r] + 4 * j];
+    b[r] = -double(m[rows[r] + 12]);
+  }
+  auto det3 = [](const double (&x)[3][3]) {
+    return x[0][0] * (x[1][1] * x[2][2] - x[1][2] * x[2][1]) -
+           x[0][1] * (x[1][0] * x[2][2] - x[1][2] * x[2][0]) +
+           x[0][2] * (x[1][0] * x[2][1] - x[1][1] * x[2][0]);
+  };
+  const double d = det3(a);
+  double scale = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) scale = fmax(scale, fabs(a[r][j]));
+  if (!(fabs(d) > 1e-12 * scale * scale * scale)) return false;
+  for (int col = 0; col < 3; ++col) {
+    double t[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int j = 0; j < 3; ++j) t[r][j] = j == col ? b[r] : a[r][j];
+    c[col] = float(det3(t) / d);
+  }
+  return true;
+}
+
 static int fill_views(dsplat_mv_plan* P, uint32_t nview, const dsplat_view_desc* views) {
   if (nview == 0 || nview > P->max_view)
     return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: view count out of range");
   std::vector<ViewDev> vd(nview);
+  bool cams_ok = true;
   uint32_t toff = 0;
   uint64_t poff = 0;
   for (uint32_t v = 0; v < nview; ++v) {
@@ -348,12 +483,18 @@ static int fill_views(dsplat_mv_plan* P, uint32_t nview, const dsplat_view_desc*
     vd[v].tile_off = toff;
     vd[v].pad0 = 0;
     vd[v].pix_off = poff;
+    if (!camera_centre(vd[v].m, vd[v].cam)) {
+      vd[v].cam[0] = vd[v].cam[1] = vd[v].cam[2] = 0.f;
+      cams_ok = false;
+    }
+    vd[v].pad1 = 0.f;
     toff += vd[v].tw * vd[v].th;
     poff += uint64_t(views[v].img_w) * views[v].img_h;
   }
   if (toff > P->max_tiles || poff > P->max_pix)
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "multiview: views exceed the plan's pixels/tiles");
   P->host_views = vd;
+  P->cams_ok = cams_ok;
   P->nview = nview;
   P->total_tiles = toff;
   P->total_pix = poff;
@@ -446,6 +587,7 @@ extern "C" int dsplat_mv_plan_destroy(dsplat_mv_plan* P) {
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
   if (P->mem) (void)hipFree(P->mem);
+  if (P->sh_mem) (void)hipFree(P->sh_mem);
   delete P;
   return DSPLAT_OK;
 }
@@ -458,6 +600,30 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
                             hipMemcpyHostToDevice, as_stream(stream)));
   if (int rc = iota_u32(as_stream(stream), P->total_tiles, P->tile_order)) return rc;
   DSPLAT_HIP(hipStreamSynchronize(as_stream(stream)));
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_mv_set_sh(dsplat_mv_plan* P, uint32_t degree, uint32_t num_coef) {
+  if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
+  if (num_coef == 0) {
+    P->sh_k = P->sh_deg = 0;
+    return DSPLAT_OK;
+  }
+  if ((num_coef != 1 && num_coef != 4 && num_coef != 9 && num_coef != 16) || degree > 3 ||
+      (degree + 1) * (degree + 1) > num_coef)
+    return dsplat_set_error(DSPLAT_ERR_ARG,
+                            "mv_set_sh: num_coef must be 0, 1, 4, 9 or 16 and degree <= 3 with "
+                            "(degree+1)^2 <= num_coef");
+  if (!P->sh_mem) {
+    const size_t items = size_t(P->n) * P->max_view;
+    char* m = nullptr;
+    DSPLAT_HIP(hipMalloc(reinterpret_cast<void**>(&m), a256(items * 12) * 2));
+    P->sh_mem = m;
+    P->rgb_item = reinterpret_cast<float*>(m);
+    P->d_rgb_item = reinterpret_cast<float*>(m + a256(items * 12));
+  }
+  P->sh_k = num_coef;
+  P->sh_deg = degree;
   return DSPLAT_OK;
 }
 
@@ -513,6 +679,50 @@ static void launch_mv_bwd(dsplat_mv_plan* P, hipStream_t s, const float* out_rgb
       P->tlast, d_img, out_rgb, target, d_alpha, out, P->misc + 3, P->tile_order);
 }
 
+template <int K, int DEG>
+static void mv_sh_launch(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* coef,
+                         float* d_coef, float* d_xyz) {
+  const uint32_t nb = div_up(P->n, 256);
+  if (d_coef)
+    mv_sh_bwd_kernel<K, DEG><<<nb, 256, 0, s>>>(P->n, P->nview, P->d_views, xyz, coef,
+                                                P->d_rgb_item, d_coef, d_xyz);
+  else
+    mv_sh_kernel<K, DEG><<<nb, 256, 0, s>>>(P->n, P->nview, P->d_views, xyz, coef,
+                                            P->rgb_item);
+}
+
+// SH colour of every item (d_coef null), or its adjoint (d_coef = dL/dsh,
+// stored; the direction part added to d_xyz).
+static void mv_sh(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* coef,
+                  float* d_coef, float* d_xyz) {
+  switch (P->sh_k * 4 + P->sh_deg) {
+    case 1 * 4 + 0: mv_sh_launch<1, 0>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 4 * 4 + 0: mv_sh_launch<4, 0>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 4 * 4 + 1: mv_sh_launch<4, 1>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 9 * 4 + 0: mv_sh_launch<9, 0>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 9 * 4 + 1: mv_sh_launch<9, 1>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 9 * 4 + 2: mv_sh_launch<9, 2>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 16 * 4 + 0: mv_sh_launch<16, 0>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 16 * 4 + 1: mv_sh_launch<16, 1>(P, s, xyz, coef, d_coef, d_xyz); break;
+    case 16 * 4 + 2: mv_sh_launch<16, 2>(P, s, xyz, coef, d_coef, d_xyz); break;
+    default: mv_sh_launch<16, 3>(P, s, xyz, coef, d_coef, d_xyz); break;
+  }
+}
+
+// Offset of the colour block in a flat parameter block: 11N, rounded up to a
+// multiple of 4 floats under SH colour (the coefficients move as 16-byte
+// words).
+static size_t colour_offset(const dsplat_mv_plan* P) {
+  const size_t o = 11 * size_t(P->n);
+  return P->sh_k ? (o + 3) & ~size_t(3) : o;
+}
+
+static size_t param_count(const dsplat_mv_plan* P) {
+  return colour_offset(P) + 3 * size_t(P->n) * (P->sh_k ? P->sh_k : 1);
+}
+
+static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
                       const float* quat, const float* opacity, const float* rgb,
                       uint32_t flags, float* out_rgb, float* out_alpha) {
@@ -522,13 +732,29 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, logit = flags & DSPLAT_OPACITY_IS_LOGIT;
   const dim3 pg(div_up(n, 256), nv);
   const float4* q4 = reinterpret_cast<const float4*>(quat);
-#define DSPLAT_MVP(A, B)                                                                    \
-  mv_project_kernel<A, B><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
-                                             rgb, P->rec, P->cnt, P->dkey)
-  if (log_scale && logit) DSPLAT_MVP(true, true);
-  else if (log_scale) DSPLAT_MVP(true, false);
-  else if (logit) DSPLAT_MVP(false, true);
-  else DSPLAT_MVP(false, false);
+  const float* col = rgb;
+  if (P->sh_k) {
+    if (!P->cams_ok)
+      return dsplat_set_error(DSPLAT_ERR_ARG,
+                              "mv_forward: SH colour needs a perspective camera in every view");
+    if (!al16(rgb))
+      return dsplat_set_error(DSPLAT_ERR_ARG, "mv_forward: SH coefficients not 16-byte aligned");
+    mv_sh(P, s, xyz, rgb, nullptr, nullptr);
+    col = P->rgb_item;
+  }
+  P->fwd_sh = P->sh_k ? rgb : nullptr;
+  P->fwd_sh_k = P->sh_k;
+  P->fwd_sh_deg = P->sh_deg;
+#define DSPLAT_MVP(A, B, C)                                                                    \
+  mv_project_kernel<A, B, C><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
+                                                col, P->rec, P->cnt, P->dkey)
+#define DSPLAT_MVP2(A, B) \
+  if (P->sh_k) DSPLAT_MVP(A, B, true); else DSPLAT_MVP(A, B, false)
+  if (log_scale && logit) { DSPLAT_MVP2(true, true); }
+  else if (log_scale) { DSPLAT_MVP2(true, false); }
+  else if (logit) { DSPLAT_MVP2(false, true); }
+  else { DSPLAT_MVP2(false, false); }
+#undef DSPLAT_MVP2
 #undef DSPLAT_MVP
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
@@ -577,12 +803,23 @@ static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const
                        uint32_t flags) {
   const uint32_t n = P->n, nv = P->nview;
   const size_t items = size_t(n) * nv;
+  if (P->sh_k && (P->fwd_sh_k != P->sh_k || P->fwd_sh_deg != P->sh_deg || !P->fwd_sh))
+    return dsplat_set_error(DSPLAT_ERR_ARG,
+                            "mv_backward: the last forward did not use the current SH colour");
+  if (P->sh_k && !al16(d_rgb))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_backward: d_sh not 16-byte aligned");
   set_word_kernel<<<1, 1, 0, s>>>(P->misc + 3, 0u);
   zero_f32_kernel<<<grid_cap(items * 2), 256, 0, s>>>(items * 2, P->d_pixxy);
   zero_f32_kernel<<<grid_cap(items * 3), 256, 0, s>>>(items * 3, P->d_conic);
   zero_f32_kernel<<<grid_cap(n), 256, 0, s>>>(n, d_op);
-  zero_f32_kernel<<<grid_cap(size_t(n) * 3), 256, 0, s>>>(size_t(n) * 3, d_rgb);
-  const GradSink out{P->d_pixxy, P->d_conic, d_op, d_rgb};
+  GradSink out{P->d_pixxy, P->d_conic, d_op, d_rgb};
+  if (P->sh_k) {
+    zero_f32_kernel<<<grid_cap(items * 3), 256, 0, s>>>(items * 3, P->d_rgb_item);
+    out.d_rgb = P->d_rgb_item;
+    out.rgb_per_item = true;
+  } else {
+    zero_f32_kernel<<<grid_cap(size_t(n) * 3), 256, 0, s>>>(size_t(n) * 3, d_rgb);
+  }
   const uint32_t ts = P->ts, th = P->threads;
   if (ts == 8) launch_mv_bwd<8, 64>(P, s, out_rgb, d_img, target, d_alpha, out);
   else if (ts == 16 && th == 64) launch_mv_bwd<16, 64>(P, s, out_rgb, d_img, target, d_alpha, out);
@@ -604,11 +841,10 @@ static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const
   else if (logit) DSPLAT_MVB(false, true);
   else DSPLAT_MVB(false, false);
 #undef DSPLAT_MVB
+  if (P->sh_k) mv_sh(P, s, xyz, P->fwd_sh, d_rgb, d_xyz);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }
-
-static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 extern "C" int dsplat_mv_forward(dsplat_mv_plan* P, void* stream, const float* xyz,
                                  const float* scale, const float* quat, const float* opacity,
@@ -659,6 +895,9 @@ extern "C" int dsplat_adam_step(void* stream, uint64_t n, float* param, const fl
 // One training step over the plan's current views on a flat parameter block
 // [xyz 3N | log-scale 3N | quat 4N | opacity logit N | rgb 3N]: forward, the
 // adjoint of sum (rgb - target)^2 over all views, and Adam on all 14N values.
+// Under SH colour the last block is the coefficients [N][K][3] starting at
+// 11N rounded up to a multiple of 4 (colour_offset), and Adam runs over the
+// whole block, padding included.
 static int mv_train_step(dsplat_mv_plan* P, hipStream_t s, float* params, float* grads,
                          float* m1, float* m2, uint32_t* d_step, const float* targets,
                          float* out_rgb, float lr, float b1, float b2, float eps) {
@@ -667,16 +906,17 @@ static int mv_train_step(dsplat_mv_plan* P, hipStream_t s, float* params, float*
   float* ls = params + 3 * n;
   float* q = params + 6 * n;
   float* op = params + 10 * n;
-  float* rgb = params + 11 * n;
+  const size_t coff = colour_offset(P), total = param_count(P);
+  float* rgb = params + coff;
   const uint32_t flags = DSPLAT_SCALE_IS_LOG | DSPLAT_OPACITY_IS_LOGIT;
   int rc = mv_forward(P, s, xyz, ls, q, op, rgb, flags, out_rgb, nullptr);
   if (rc) return rc;
   rc = mv_backward(P, s, xyz, ls, q, op, out_rgb, nullptr, targets, nullptr, grads,
-                   grads + 3 * n, grads + 6 * n, grads + 10 * n, grads + 11 * n, flags);
+                   grads + 3 * n, grads + 6 * n, grads + 10 * n, grads + coff, flags);
   if (rc) return rc;
   adam_tick_kernel<<<1, 1, 0, s>>>(d_step);
-  adam_kernel<<<grid_cap(14 * n), 256, 0, s>>>(14 * n, params, grads, m1, m2, lr, b1, b2, eps,
-                                               d_step);
+  adam_kernel<<<grid_cap(total), 256, 0, s>>>(total, params, grads, m1, m2, lr, b1, b2, eps,
+                                              d_step);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }
@@ -690,6 +930,8 @@ extern "C" int dsplat_mv_train_step(dsplat_mv_plan* P, void* stream, float* para
   if (P->nview == 0) return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: no views set");
   if (!al16(params + 6 * size_t(P->n)) || !al16(grads + 6 * size_t(P->n)))
     return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: quat block not 16-byte aligned");
+  if (P->sh_k && (!al16(params) || !al16(grads)))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "mv_train_step: parameter block not 16-byte aligned");
   return mv_train_step(P, as_stream(stream), params, grads, m1, m2, d_step, targets, out_rgb, lr,
                        beta1, beta2, eps);
 }
@@ -727,7 +969,7 @@ extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const f
     if (!rc)
       rc = mv_backward(P, P->cap_stream, xyz, scale, quat, opacity, out_rgb, nullptr, targets,
                        nullptr, grads, grads + 3 * n, grads + 6 * n, grads + 10 * n,
-                       grads + 11 * n, flags);
+                       grads + colour_offset(P), flags);
   } else {
     rc = (params && grads && m1 && m2 && d_step && targets)
              ? mv_train_step(P, P->cap_stream, params, grads, m1, m2, d_step, targets, out_rgb,

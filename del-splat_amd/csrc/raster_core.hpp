@@ -372,7 +372,8 @@ struct GradSink {
   float* d_pixxy;   // [items, 2]
   float* d_conic;   // [items, 3]
   float* d_op;      // [gaussians]
-  float* d_rgb;     // [gaussians, 3]
+  float* d_rgb;     // [gaussians, 3], or [items, 3] with rgb_per_item
+  bool rgb_per_item = false;  // per-view colours (SH colour of a batched plan)
 };
 
 // LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
@@ -599,7 +600,8 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       if (sum[5] != 0.f) atomicAdd(&out.d_op[g], sum[5]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        if (sum[6 + c] != 0.f) atomicAdd(&out.d_rgb[3 * size_t(g) + c], sum[6 + c]);
+        if (sum[6 + c] != 0.f)
+          atomicAdd(&out.d_rgb[3 * size_t(out.rgb_per_item ? p : g) + c], sum[6 + c]);
       }
     }
     __syncthreads();

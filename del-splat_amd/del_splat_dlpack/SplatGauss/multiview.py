@@ -54,6 +54,7 @@ class MultiViewPlan:
         # plan's own buffer (forwards given no out_alpha, training steps)
         self._last_alpha = None
         self._graph_alpha = None
+        self.sh_degree, self.sh_coef = 0, 0  # plain rgb until set_sh
         self.set_views(views)
 
     # ------------------------------------------------------------- views --
@@ -73,6 +74,37 @@ class MultiViewPlan:
         self.pix_off = [int(x) for x in po]
         self.tile_off = [int(x) for x in to]
         self.graph_kind = None
+
+    def set_sh(self, degree, num_coef=16):
+        """Colour from spherical-harmonic coefficients [N, num_coef, 3] seen
+        from each view's camera centre (num_coef = 0: one rgb per Gaussian).
+        The forward's colour argument and the backward's colour gradient then
+        have that shape."""
+        call("dsplat_mv_set_sh", self._plan, int(degree), int(num_coef))
+        self.sh_degree, self.sh_coef = (int(degree), int(num_coef)) if num_coef else (0, 0)
+        self.graph_kind = None
+
+    def param_layout(self):
+        """Offsets of the flat training block: dict of name -> (start, stop)
+        for xyz, log_scale, quat, opacity (logit) and colour (rgb, or the SH
+        coefficients, which start at 11N rounded up to a multiple of 4)."""
+        n = self.num_pnt
+        c0 = 11 * n
+        if self.sh_coef:
+            c0 = (c0 + 3) // 4 * 4
+        k = self.sh_coef or 1
+        return dict(xyz=(0, 3 * n), log_scale=(3 * n, 6 * n), quat=(6 * n, 10 * n),
+                    opacity=(10 * n, 11 * n), colour=(c0, c0 + 3 * k * n))
+
+    def pack_params(self, xyz, log_scale, quat, opacity_logit, colour):
+        """Flat training block (float32 tensor on the plan's device; padding 0)."""
+        lay = self.param_layout()
+        flat = torch.zeros(lay["colour"][1], device=self.device, dtype=torch.float32)
+        for name, t in (("xyz", xyz), ("log_scale", log_scale), ("quat", quat),
+                        ("opacity", opacity_logit), ("colour", colour)):
+            a, b = lay[name]
+            flat[a:b] = torch.as_tensor(t, dtype=torch.float32).reshape(-1).to(self.device)
+        return flat
 
     def split(self, flat, channels):
         """Concatenated [total_pix, C] buffer -> list of per-view [H, W, C]."""
@@ -100,9 +132,10 @@ class MultiViewPlan:
                  d_alpha=None, flags=_native.DSPLAT_SCALE_IS_LOG):
         n = self.num_pnt
         dev = self.device
+        colour_shape = (n, self.sh_coef, 3) if self.sh_coef else (n, 3)
         g = dict(xyz=torch.empty((n, 3), device=dev), scale=torch.empty((n, 3), device=dev),
                  quat=torch.empty((n, 4), device=dev), opacity=torch.empty(n, device=dev),
-                 rgb=torch.empty((n, 3), device=dev))
+                 rgb=torch.empty(colour_shape, device=dev))
         call("dsplat_mv_backward", self._plan, stream_ptr(dev), ptr(xyz), ptr(scale), ptr(quat),
              ptr(opacity), ptr(out_rgb), ptr(d_img), ptr(target), ptr(d_alpha), ptr(g["xyz"]),
              ptr(g["scale"]), ptr(g["quat"]), ptr(g["opacity"]), ptr(g["rgb"]), flags)

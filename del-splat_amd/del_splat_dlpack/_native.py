@@ -55,6 +55,7 @@ _SIGS = {
     "dsplat_gauss_rasterize_bwd_l2": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_gauss_project_bwd": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_sh_to_rgb": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "dsplat_mv_set_sh": (_int, [_vp, _u32, _u32]),
     "dsplat_sh_to_rgb_bwd": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
     "dsplat_circle_pnt2splat3_to_pnt2splat2": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _u32, _u32, _f32]),
     "dsplat_circle_tile2idx_idx2pnt_ws": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _sz, _vp]),

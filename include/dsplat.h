@@ -362,6 +362,20 @@ int dsplat_mv_plan_destroy(dsplat_mv_plan* plan);
 int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
                         const dsplat_view_desc* views);
 
+/* SH colour for the plan's next frames (no counterpart in the reference):
+ * with num_coef in {1, 4, 9, 16} and (degree+1)^2 <= num_coef, the colour
+ * argument of dsplat_mv_forward (and the colour block of the training
+ * parameter block) holds spherical-harmonic coefficients [N][num_coef][3]
+ * (16-byte aligned), evaluated per view from the camera centre of its
+ * world2ndc as dsplat_sh_to_rgb does; the colour gradient of
+ * dsplat_mv_backward is then dL/dsh [N][num_coef][3], and the direction part
+ * of dL/dxyz is included. The backward reads the coefficients the last
+ * forward was given. num_coef = 0 returns to one rgb per Gaussian. The first
+ * call allocates 24 bytes per (view, Gaussian) item of the plan (hipMalloc).
+ * Every view must be a perspective camera (its centre is solved from the
+ * matrix when views are set). */
+int dsplat_mv_set_sh(dsplat_mv_plan* plan, uint32_t degree, uint32_t num_coef);
+
 /* Host query: view count, total pixels/tiles and the per-view pixel and
  * global-tile offsets (arrays of num_view+1, may be null). */
 int dsplat_mv_plan_info(const dsplat_mv_plan* plan, uint32_t* num_view, uint64_t* total_pix,
@@ -386,7 +400,8 @@ int dsplat_mv_forward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
 /* Adjoint of the last forward, summed over views. dL/drgb comes from d_img
  * [pix,3] or, when d_img is null, from the loss sum (out_rgb - target)^2 with
  * target [pix,3]. d_alpha [pix] may be null. Overwrites d_pnt2xyz [N,3],
- * d_pnt2scale [N,3], d_pnt2quat [N,4], d_pnt2opacity [N], d_pnt2rgb [N,3]. */
+ * d_pnt2scale [N,3], d_pnt2quat [N,4], d_pnt2opacity [N], d_pnt2rgb [N,3]
+ * (under SH colour: dL/dsh [N][K][3]). */
 int dsplat_mv_backward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
                        const float* pnt2scale, const float* pnt2quat, const float* pnt2opacity,
                        const float* out_rgb, const float* d_img, const float* target,
@@ -402,15 +417,19 @@ int dsplat_adam_step(void* stream, uint64_t n, float* param, const float* grad, 
 /* One training step on the plan's views over the flat parameter block
  * params [14N] = [xyz 3N | log-scale 3N | quat 4N | opacity logit N | rgb 3N]
  * (N even): forward into out_rgb, adjoint of sum (out_rgb - targets)^2 into
- * grads [14N], Adam on all 14N values (moments m1, m2 [14N]). */
+ * grads [14N], Adam on all 14N values (moments m1, m2 [14N]). Under SH colour
+ * (dsplat_mv_set_sh) the block is [xyz | log-scale | quat | opacity logit |
+ * padding to a multiple of 4 floats | sh 3KN], i.e. the coefficients start at
+ * 11N rounded up to a multiple of 4, and Adam covers the whole block. */
 int dsplat_mv_train_step(dsplat_mv_plan* plan, void* stream, float* params, float* grads,
                          float* m1, float* m2, uint32_t* d_step, const float* targets,
                          float* out_rgb, float lr, float beta1, float beta2, float eps);
 
 /* Captures one forward (kind 0: the dsplat_mv_forward arguments), one
  * training step (kind 1: the dsplat_mv_train_step arguments) or one forward
- * + L2 adjoint (kind 2: forward arguments, targets, and grads [14N] laid out
- * as the training parameter block) into the plan's hipGraph;
+ * + L2 adjoint (kind 2: forward arguments, targets, and grads laid out as the
+ * training parameter block, 14N values or the SH layout) into the plan's
+ * hipGraph;
  * dsplat_mv_graph_launch replays it on any stream. The pointers are baked
  * into the graph. */
 int dsplat_mv_graph_capture(dsplat_mv_plan* plan, uint32_t kind, const float* pnt2xyz,

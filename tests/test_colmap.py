@@ -269,3 +269,20 @@ def test_gaussians_from_points_lattice():
                           np.zeros((4, 3), np.uint8), np.zeros(4), np.zeros(5, np.int64),
                           np.zeros((0, 2), np.int32))
     assert np.isfinite(colmap.gaussians_from_points(dup)["log_scale"]).all()
+
+
+def test_camera_center_is_the_eye(recon):
+    """camera.camera_center recovers the eye of look-at cameras and COLMAP's
+    centre -R^T t of registered images from their world->ndc matrices."""
+    from del_splat_dlpack import camera
+    rng = np.random.default_rng(12)
+    for _ in range(10):
+        eye = rng.standard_normal(3) * 3
+        m = camera.world2ndc(eye, (0.0, 0.0, 0.0), (320, 240))
+        np.testing.assert_allclose(camera.camera_center(m), eye, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(camera.camera_center(camera.to_col_major(m)), eye, rtol=0,
+                                   atol=5e-5)
+    cams, images, _, _ = recon
+    for (m16, _), im in zip(colmap.views(cams, images), images):
+        r = colmap.qvec_to_rotation(im.qvec)
+        np.testing.assert_allclose(camera.camera_center(m16), -r.T @ im.tvec, rtol=0, atol=5e-5)

@@ -124,7 +124,9 @@ def test_render_with_sh_colour_gradients(oracle):
     leaf, sh, rgb = run(True)
     leaf0, _, rgb0 = run(False)
     d_rgb = rgb.grad.cpu().numpy()
-    np.testing.assert_array_equal(d_rgb, rgb0.grad.cpu().numpy())
+    # the two renders' colour gradients agree up to the order of the
+    # adjoint's float atomics
+    assert max_rel(d_rgb, rgb0.grad.cpu().numpy()) < 1e-5
     o_sh, o_xyz = oracle.sh_to_rgb_bwd(s["xyz"], sh_np, eye, 3, d_rgb)
     assert max_rel(sh.grad.cpu().numpy(), o_sh) < 1e-5
     sh_part = leaf["xyz"].grad.cpu().numpy() - leaf0["xyz"].grad.cpu().numpy()
