@@ -60,17 +60,34 @@ def tile2idx_idx2pnt(img_shape, tile_size, pnt2pixxydepth, pnt2pixaabb):
     return bin_phases(dev, n, tw * th, count, fill)
 
 
-def _bin_with_capacity(img_shape, tile_size, n, pxd, bb, cap, kbuf):
-    dev = pxd.device
+def bin_gauss_capacity(img_shape, tile_size, pnt2pixxydepth, pnt2pixaabb, capacity,
+                       d_num_idx=None):
+    """Binning + stable sort with no host synchronisation
+    (dsplat_gauss_tile2idx_idx2pnt_ws): returns (tile2idx [T+1], idx2pnt
+    [capacity], d_num_idx), where d_num_idx[0] is K on the device. When
+    K <= capacity the lists equal tile2idx_idx2pnt's (idx2pnt[:K]); when
+    K > capacity every list is empty and K tells the capacity to use."""
+    require_gpu(pnt2pixaabb, "pnt2pixaabb")
+    dev = pnt2pixaabb.device
+    n = pnt2pixaabb.shape[0]
+    pxd = pnt2pixxydepth.contiguous()
+    bb = pnt2pixaabb.contiguous()
     tw, th = tile_shape(img_shape, tile_size)
-    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, cap)
+    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, capacity)
     ws = torch.empty(ws_bytes, device=dev, dtype=torch.uint8)
     tile2idx = torch.empty(tw * th + 1, device=dev, dtype=torch.int32)
-    idx2pnt = torch.empty(max(cap, 1), device=dev, dtype=torch.int32)
+    idx2pnt = torch.empty(max(capacity, 1), device=dev, dtype=torch.int32)
+    if d_num_idx is None:
+        d_num_idx = torch.empty(1, device=dev, dtype=torch.int32)
     call("dsplat_gauss_tile2idx_idx2pnt_ws", stream_ptr(dev), img_shape[0], img_shape[1],
-         tile_size, n, ptr(pxd), ptr(bb), ptr(tile2idx), ptr(idx2pnt), cap, ptr(ws), ws_bytes,
-         ptr(kbuf))
-    return tile2idx, idx2pnt[:cap]
+         tile_size, n, ptr(pxd), ptr(bb), ptr(tile2idx), ptr(idx2pnt), capacity, ptr(ws),
+         ws_bytes, ptr(d_num_idx))
+    return tile2idx, idx2pnt[:capacity], d_num_idx
+
+
+def _bin_with_capacity(img_shape, tile_size, n, pxd, bb, cap, kbuf):
+    tile2idx, idx2pnt, _ = bin_gauss_capacity(img_shape, tile_size, pxd, bb, cap, kbuf)
+    return tile2idx, idx2pnt
 
 
 def rasterize_pnt2splat2(img_shape, pnt2pixxydepth, pnt2pixconvinv, pnt2pixaabb, pnt2opacity,
