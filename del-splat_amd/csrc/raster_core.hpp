@@ -335,31 +335,59 @@ __device__ inline void wave_sums_to_last(float (&v)[N]) {
   for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x143, 0xC>(v[c]);
 }
 
-// Sums of 8 values over the wave, as a reduce-scatter: three exchange
-// levels inside each 8-lane group (half-row mirror, quad swaps) leave lane r
-// with its group's sum of value r & 7 (3 + 6 + ... = 21 instructions for all
-// eight values instead of 8 full reductions), then a rotate by 8 within the
-// row and two cross-row exchanges complete the sums. Every lane returns the
-// total of value (lane & 7).
+// v_permlane32_swap: the upper 32 lanes of a trade places with the lower 32
+// lanes of b (one instruction moves both directions).
+__device__ inline void swap_wave_halves(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                  false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+
+// v_permlane16_swap: the odd 16-lane rows of a trade places with the even
+// rows of b.
+__device__ inline void swap_wave_rows(float& a, float& b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false,
+                                                  false);
+  a = __uint_as_float(r[0]);
+  b = __uint_as_float(r[1]);
+}
+
+// Which of the eight values wave_sum8_scatter leaves in lane l:
+// 4 b3 + 2 b4 + b5, b_k = bit k of l (every lane holds one total).
+__device__ inline int sum8_slot(int lane) {
+  return ((lane >> 3) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 5) & 1);
+}
+
+// Sums of 8 values over the wave, as a reduce-scatter. Value pairs are
+// exchanged between the wave halves (v_permlane32_swap: one swap and one add
+// per pair leave the first value's partial sums in the lower half and the
+// second's in the upper), then between 16-lane rows (v_permlane16_swap), then
+// between the two 8-lane halves of each row (a row rotate by 8 with a
+// select); three DPP levels finish the sums inside each 8-lane group: 18
+// VALU instructions for the eight totals, no LDS. Lane l returns the total of
+// value sum8_slot(l).
 __device__ inline float wave_sum8_scatter(const float (&v)[8], int lane) {
-  const bool b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
-  float w[4];
+  float s[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const float keep = b2 ? v[c + 4] : v[c], send = b2 ? v[c] : v[c + 4];
-    w[c] = keep + dpp_mov<0x141>(send);  // row_half_mirror: lane 7 - i of the group
+  for (int k = 0; k < 4; ++k) {
+    float a = v[2 * k], b = v[2 * k + 1];
+    swap_wave_halves(a, b);
+    s[k] = a + b;
   }
-  float x[2];
+  float t[2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const float keep = b1 ? w[c + 2] : w[c], send = b1 ? w[c] : w[c + 2];
-    x[c] = keep + dpp_mov<0x4E>(send);  // quad_perm [2,3,0,1]
+  for (int k = 0; k < 2; ++k) {
+    float a = s[2 * k], b = s[2 * k + 1];
+    swap_wave_rows(a, b);
+    t[k] = a + b;
   }
-  float y = (b0 ? x[1] : x[0]) + dpp_mov<0xB1>(b0 ? x[0] : x[1]);  // quad_perm [1,0,3,2]
-  y += dpp_mov<0x128>(y);  // row_ror:8
-  y += __shfl_xor(y, 16, 64);
-  y += __shfl_xor(y, 32, 64);
-  return y;
+  const bool b3 = lane & 8;
+  float u = (b3 ? t[1] : t[0]) + dpp_mov<0x128>(b3 ? t[0] : t[1]);  // row_ror:8
+  u += dpp_mov<0xB1>(u);  // quad_perm [1,0,3,2]
+  u += dpp_mov<0x4E>(u);  // quad_perm [2,3,0,1]
+  u += dpp_mov<0x141>(u);  // row_half_mirror
+  return u;
 }
 
 // Reduced per list entry: S_x, S_y, S_xx, S_xy, S_yy (moments of dL/dq over
@@ -563,13 +591,16 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const float tot = wave_sum8_scatter(v8, lane);
       float vb[1] = {v[8]};
       wave_sums_to_last(vb);
+      // lanes 0, 8, ..., 56 hold the eight totals, lane 63 the ninth
+      const bool slot_lane = (lane & 7) == 0;
       if (W == 1 || S > 1) {
-        if (lane < 8) acc[lane][j] = tot;
+        if (slot_lane) acc[sum8_slot(lane)][j] = tot;
         if (lane == 63) acc[8][j] = vb[0];
       } else {
         // one atomic with a per-lane slot: a lane-uniform address would be
         // expanded by the compiler into a loop over the active lanes
-        if (lane < 8 || lane == 63) atomicAdd(&acc[lane < 8 ? lane : 8][j], lane < 8 ? tot : vb[0]);
+        if (slot_lane || lane == 63)
+          atomicAdd(&acc[slot_lane ? sum8_slot(lane) : 8][j], slot_lane ? tot : vb[0]);
       }
     }
     __syncthreads();
