@@ -50,6 +50,7 @@ bin_count_kernel(uint32_t n, Boxes boxes, const float* __restrict__ pixxydepth, 
     cnt[i] = v;
     if (dkey) dkey[i] = depth_key(pixxydepth[3 * i + 2]);
   }
+  if (!partial) return;  // uniform: K comes from the scan of the ordered counts
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += uint32_t(__shfl_xor(int(v), off, 64));
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
@@ -185,13 +186,15 @@ template <typename Boxes>
 int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
               uint32_t tw, uint32_t th, const PointLayout& L, uint32_t* d_num_idx) {
   if (n == 0) {
-    set_u32_one<<<1, 1, 0, s>>>(d_num_idx, 0u);
+    if (d_num_idx) set_u32_one<<<1, 1, 0, s>>>(d_num_idx, 0u);
   } else {
-    // per-workgroup sums go to off[] (rewritten by the order phase)
+    // per-workgroup sums go to off[] (rewritten by the order phase); with no
+    // d_num_idx, K is left to the order phase's scan
     const uint32_t nb = div_up(n, 256);
     bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt,
-                                               pixxydepth ? L.dkey : nullptr, L.off);
-    sum_partials_kernel<<<1, 256, 0, s>>>(nb, L.off, d_num_idx);
+                                               pixxydepth ? L.dkey : nullptr,
+                                               d_num_idx ? L.off : nullptr);
+    if (d_num_idx) sum_partials_kernel<<<1, 256, 0, s>>>(nb, L.off, d_num_idx);
   }
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
@@ -199,9 +202,13 @@ int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, u
 
 // Phase 2: stable depth order of the Gaussians (values generated as iota by
 // the first pass; the last pass also gathers the counts into depth order),
-// then the exclusive scan of the counts = each Gaussian's first pair slot.
-inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L) {
-  if (n == 0) return DSPLAT_OK;
+// then the exclusive scan of the counts = each Gaussian's first pair slot;
+// d_total (may be null) receives the scan's total, K.
+inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* d_total = nullptr) {
+  if (n == 0) {
+    if (d_total) set_u32_one<<<1, 1, 0, s>>>(d_total, 0u);
+    return DSPLAT_OK;
+  }
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
   int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
@@ -209,7 +216,7 @@ inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L) {
   if (rc) return rc;
   if (in_alt)  // 31 key bits take 4 passes: the result is in place
     return dsplat_set_error(DSPLAT_ERR_ARG, "binning: depth order in the alternate buffer");
-  return exclusive_scan(s, n, nullptr, L.off, L.off, L.scan_ws, nullptr);
+  return exclusive_scan(s, n, nullptr, L.off, L.off, L.scan_ws, d_total);
 }
 
 // Phase 3: pairs in depth order, stable sort by tile id, tile ranges. Writes
@@ -254,8 +261,9 @@ int bin_and_sort(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth
   const PairLayout Q = pair_layout(static_cast<char*>(ws) + L.total, capacity);
   if (ws_bytes < L.total + Q.total)
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "binning: workspace too small");
-  int rc = bin_count(s, n, boxes, pixxydepth, ts, tw, th, L, d_num_idx);
-  if (!rc) rc = bin_order(s, n, L);
+  // K from the scan of the depth-ordered counts (no separate sum)
+  int rc = bin_count(s, n, boxes, pixxydepth, ts, tw, th, L, nullptr);
+  if (!rc) rc = bin_order(s, n, L, d_num_idx);
   if (!rc) rc = bin_fill(s, n, boxes, ts, tw, th, L, d_num_idx, tile2idx, idx2pnt, capacity, Q);
   return rc;
 }
