@@ -592,9 +592,20 @@ extern "C" int dsplat_mv_plan_destroy(dsplat_mv_plan* P) {
   return DSPLAT_OK;
 }
 
+// A captured graph bakes in view counts, tile totals, grid sizes and the SH
+// mode; changing any of them drops it (the next dsplat_mv_graph_launch then
+// reports that nothing is captured instead of replaying stale launches).
+static void drop_graph(dsplat_mv_plan* P) {
+  if (P->exec) {
+    (void)hipGraphExecDestroy(P->exec);
+    P->exec = nullptr;
+  }
+}
+
 extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num_view,
                                    const dsplat_view_desc* views) {
   if (!P || !views) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan/views");
+  drop_graph(P);
   if (int rc = fill_views(P, num_view, views)) return rc;
   DSPLAT_HIP(hipMemcpyAsync(P->d_views, P->host_views.data(), sizeof(ViewDev) * num_view,
                             hipMemcpyHostToDevice, as_stream(stream)));
@@ -605,6 +616,7 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
 
 extern "C" int dsplat_mv_set_sh(dsplat_mv_plan* P, uint32_t degree, uint32_t num_coef) {
   if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
+  drop_graph(P);
   if (num_coef == 0) {
     P->sh_k = P->sh_deg = 0;
     return DSPLAT_OK;

@@ -358,7 +358,8 @@ int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64_t max_pixe
 int dsplat_mv_plan_destroy(dsplat_mv_plan* plan);
 
 /* SYNC. Sets the views of the next frames (host array of num_view). Images
- * are concatenated in view order: view v's pixels start at pix_offsets[v]. */
+ * are concatenated in view order: view v's pixels start at pix_offsets[v].
+ * Drops a captured graph (its launches were sized for the old views). */
 int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
                         const dsplat_view_desc* views);
 
@@ -373,7 +374,7 @@ int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
  * forward was given. num_coef = 0 returns to one rgb per Gaussian. The first
  * call allocates 24 bytes per (view, Gaussian) item of the plan (hipMalloc).
  * Every view must be a perspective camera (its centre is solved from the
- * matrix when views are set). */
+ * matrix when views are set). Drops a captured graph. */
 int dsplat_mv_set_sh(dsplat_mv_plan* plan, uint32_t degree, uint32_t num_coef);
 
 /* Host query: view count, total pixels/tiles and the per-view pixel and

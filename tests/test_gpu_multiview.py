@@ -116,6 +116,33 @@ def test_graph_replay_forward():
     plan.close()
 
 
+def test_changing_views_or_colour_drops_the_graph():
+    """A captured graph is sized for its views; after set_views or set_sh a
+    replay raises instead of running stale launches, and a new capture
+    works."""
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
+    s, views = _setup(n=4000, seed=43)
+    n = s["xyz"].shape[0]
+    args = [_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb")]
+    plan = MultiViewPlan(n, views, tile_size=16)
+    out_rgb, out_a = plan.new_image(3), plan.new_image(1)
+    plan.capture_forward(*args, out_rgb, out_a)
+    plan.replay()
+    plan.set_views(views[::-1])  # other image offsets than the captured launches
+    with pytest.raises(ValueError, match="nothing captured"):
+        plan.replay()
+    big_rgb, big_a = plan.new_image(3), plan.new_image(1)
+    plan.capture_forward(*args, big_rgb, big_a)
+    plan.replay()
+    torch.cuda.synchronize()
+    eager, _ = plan.forward(*args)
+    np.testing.assert_array_equal(big_rgb.cpu().numpy(), eager.cpu().numpy())
+    plan.set_sh(0, 0)
+    with pytest.raises(ValueError, match="nothing captured"):
+        plan.replay()
+    plan.close()
+
+
 def test_capacity_overflow_and_growth():
     from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan, ensure_capacity
     s, views = _setup(n=5000, seed=43)
