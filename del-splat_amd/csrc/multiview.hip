@@ -273,6 +273,7 @@ mv_raster_fwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
     const uint32_t q = next_work_item(counter);
     if (q >= total_tiles) break;
     const uint32_t g = tile_order[q];  // longest walks first
+    if (g >= total_tiles) continue;    // uniform; the plan writes only permutations
     const uint32_t v = view_of_tile(views, nview, g);
     const ViewDev& vd = views[v];
     const uint32_t local = g - vd.tile_off;
@@ -297,6 +298,7 @@ mv_raster_bwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
     const uint32_t q = next_work_item(counter);
     if (q >= total_tiles) break;
     const uint32_t g = tile_order[q];  // longest walks first
+    if (g >= total_tiles) continue;    // uniform; the plan writes only permutations
     const uint32_t v = view_of_tile(views, nview, g);
     const ViewDev& vd = views[v];
     const uint32_t local = g - vd.tile_off;

@@ -17,6 +17,7 @@ gauss_raster_fwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw, Loader ld,
                         float* __restrict__ pix2tlast, const uint32_t* __restrict__ tile_order,
                         uint32_t* __restrict__ tile_cost) {
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+  if (tile >= gridDim.x) return;  // uniform: an order entry outside the grid is skipped
   raster_tile_fwd<TS, NT, Loader>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile],
                                   tile2idx[tile + 1], ld, idx2pnt, pix2rgb, pix2alpha, pix2nwalk,
                                   pix2tlast, tile_cost, tile);
@@ -34,6 +35,7 @@ gauss_raster_bwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw,
                         float* __restrict__ d_conic, float* __restrict__ d_op,
                         float* __restrict__ d_rgb, const uint32_t* __restrict__ tile_order) {
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
+  if (tile >= gridDim.x) return;  // uniform: an order entry outside the grid is skipped
   const GradSink out{d_pixxy, d_conic, d_op, d_rgb};
   raster_tile_bwd<TS, NT>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile + 1], splat2,
                           idx2pnt, pix2nwalk, pix2tlast, d_img, pix2rgb, target, d_alpha, 0u,

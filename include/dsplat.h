@@ -181,7 +181,9 @@ int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
 /* Tile schedule (no counterpart in the reference; results do not depend on
  * it). The _ordered raster variants process tile tile_order[i] as their i-th
  * workgroup (tile_order: u32[T], a permutation of 0..T-1; null = raster
- * order). dsplat_gauss_rasterize_fwd_ordered also writes tile_cost[t] = the
+ * order; an entry >= T is skipped, and a repeated entry renders its tile
+ * twice, so only a permutation gives the full image).
+ * dsplat_gauss_rasterize_fwd_ordered also writes tile_cost[t] = the
  * longest walk (pix2nwalk) in tile t (may be null), and dsplat_tile_order
  * turns such costs into the descending-cost order. Launching the longest
  * walks first shortens the tail of a launch: a frame's forward costs order

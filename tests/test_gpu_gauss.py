@@ -430,6 +430,16 @@ def test_tile_schedule_orders_and_results():
                           tile_order=SG.tile_order(tc))
     for a, b in zip(g0, g1):
         assert max_rel(b.cpu().numpy(), a.cpu().numpy()) <= 1e-5
+    # order entries beyond the grid are skipped: every other tile renders as in
+    # raster order
+    bad = perm.clone()
+    skipped = bad[:10].cpu().numpy().copy()
+    bad[:10] = torch.arange(T + 3, T + 13, device=DEV, dtype=torch.int32)
+    got = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, tile_order=bad)
+    tile_of_pix = ((np.arange(shape[1])[:, None] // ts) * tw + np.arange(shape[0])[None, :] // ts)
+    keep = ~np.isin(tile_of_pix, skipped)
+    np.testing.assert_array_equal(got[0].cpu().numpy()[keep], ref[0].cpu().numpy()[keep])
+    np.testing.assert_array_equal(got[2].cpu().numpy()[keep], ref[2].cpu().numpy()[keep])
 
 
 def test_headline_frame_matches_oracle(oracle):
