@@ -85,10 +85,6 @@ def bin_gauss_capacity(img_shape, tile_size, pnt2pixxydepth, pnt2pixaabb, capaci
     return tile2idx, idx2pnt[:capacity], d_num_idx
 
 
-def _bin_with_capacity(img_shape, tile_size, n, pxd, bb, cap, kbuf):
-    tile2idx, idx2pnt, _ = bin_gauss_capacity(img_shape, tile_size, pxd, bb, cap, kbuf)
-    return tile2idx, idx2pnt
-
 
 def rasterize_pnt2splat2(img_shape, pnt2pixxydepth, pnt2pixconvinv, pnt2pixaabb, pnt2opacity,
                          pnt2rgb, tile_size, tile2idx, idx2pnt):

@@ -117,7 +117,7 @@ def test_binning_one_call_and_count_only(oracle):
                  shape[1], 16, 4000, aabb.data_ptr(), kbuf.data_ptr())
     assert int(kbuf[0].item()) == len(o_i2p)
     for cap in (len(o_i2p), len(o_i2p) + 1000):
-        t2i, i2p = SG._bin_with_capacity(shape, 16, 4000, pxd, aabb, cap, kbuf)
+        t2i, i2p, _ = SG.bin_gauss_capacity(shape, 16, pxd, aabb, cap, kbuf)
         assert int(kbuf[0].item()) == len(o_i2p)
         np.testing.assert_array_equal(u32(t2i), o_t2i)
         np.testing.assert_array_equal(u32(i2p)[:len(o_i2p)], o_i2p)
@@ -170,7 +170,7 @@ def test_capacity_overflow_reports_k(oracle):
     s, m = small_scene(2000, 7, (128, 128))
     pxd, conic, aabb = _project(s, m, (128, 128))
     kbuf = torch.zeros(4, device=DEV, dtype=torch.int32)
-    t2i, i2p = SG._bin_with_capacity((128, 128), 16, 2000, pxd, aabb, 10, kbuf)
+    t2i, i2p, _ = SG.bin_gauss_capacity((128, 128), 16, pxd, aabb, 10, kbuf)
     o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), (128, 128), 16)
     assert int(kbuf[0].item()) == len(o_i2p) > 10
     assert int(t2i.abs().sum().item()) == 0  # nothing binned on overflow
