@@ -7,8 +7,11 @@ import time
 
 import torch
 
+# (tile size, workgroup threads; 0 = the default for the tile size). The
+# batched plan runs its forward and its adjoint at the same size, and both
+# kernels exist for every entry, so one list serves both modes.
 CANDIDATES = ((8, 0), (16, 64), (16, 128), (16, 256), (32, 256), (32, 512), (32, 1024))
-CANDIDATES_BWD = ((8, 0), (16, 64), (16, 128), (16, 256), (32, 256))
+CANDIDATES_BWD = CANDIDATES
 CACHE = os.environ.get("DSPLAT_AUTOTUNE_CACHE",
                        os.path.join(os.path.expanduser("~"), ".cache", "dsplat_autotune.json"))
 
