@@ -38,7 +38,7 @@ struct CircleBoxes {
 template <typename Boxes>
 __global__ void __launch_bounds__(256)
 bin_count_kernel(uint32_t n, Boxes boxes, const float* __restrict__ pixxydepth, uint32_t ts,
-                 uint32_t tw, uint32_t th, uint32_t* __restrict__ cnt,
+                 uint32_t tw, uint32_t th, uint2* __restrict__ rect,
                  uint32_t* __restrict__ dkey, uint32_t* __restrict__ partial) {
   __shared__ uint32_t wsum[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,7 +47,7 @@ bin_count_kernel(uint32_t n, Boxes boxes, const float* __restrict__ pixxydepth, 
     const float4 b = boxes(i);
     const TileRect r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
     v = uint32_t(r.count());
-    cnt[i] = v;
+    rect[i] = pack_rect(r);
     if (dkey) dkey[i] = depth_key(pixxydepth[3 * i + 2]);
   }
   if (!partial) return;  // uniform: K comes from the scan of the ordered counts
@@ -74,14 +74,14 @@ sum_partials_kernel(uint32_t nb, const uint32_t* __restrict__ partial, uint32_t*
 // Pairs of the j-th Gaussian in depth order at off[j]: tile ids ix-outer,
 // iy-inner (tile_acceleration.h:29-50), value = Gaussian index. n_eff = K if
 // K <= capacity else 0 (overflow: nothing is written and the later phases
-// see no pairs); workgroup 0 stores it for them.
-template <typename Boxes>
+// see no pairs); workgroup 0 stores it for them. The rectangles come in depth
+// order from the sort (read in sequence, no gather of boxes).
 __global__ void __launch_bounds__(256)
-bin_fill_kernel(uint32_t n, Boxes boxes, const uint32_t* __restrict__ order,
-                const uint32_t* __restrict__ off, const uint32_t* __restrict__ d_k,
-                uint32_t capacity, uint32_t* __restrict__ n_eff_out, uint32_t ts, uint32_t tw,
-                uint32_t th, uint32_t* __restrict__ tile_of_pair,
-                uint32_t* __restrict__ pnt_of_pair) {
+bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
+                const uint2* __restrict__ rect_sorted, const uint32_t* __restrict__ off,
+                const uint32_t* __restrict__ d_k, uint32_t capacity,
+                uint32_t* __restrict__ n_eff_out, uint32_t tw,
+                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ pnt_of_pair) {
   __shared__ ExpandLds lds[4];
   const uint32_t k = *d_k;
   const uint32_t n_eff = k <= capacity ? k : 0u;
@@ -93,8 +93,7 @@ bin_fill_kernel(uint32_t n, Boxes boxes, const uint32_t* __restrict__ order,
   TileRect r{0, 0, 0, 0};
   if (valid) {
     g = order[j];
-    const float4 b = boxes(g);
-    r = tile_rect(b.x, b.y, b.z, b.w, ts, tw, th);
+    r = unpack_rect(rect_sorted[j]);  // gathered into depth order by the sort
     o = off[j];
   }
   wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair);
@@ -133,7 +132,8 @@ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 // Workspace of the per-Gaussian phases (count, order), sized by N, and of
 // the per-pair phase (fill, tile sort), sized by the pair capacity.
 struct PointLayout {
-  uint32_t* cnt;       // [N]
+  uint2* rect;         // [N] tile rectangles (pack_rect)
+  uint2* rect_sorted;  // [N] the same in depth order
   uint32_t* dkey;      // [N]
   uint32_t* order;     // [N] Gaussians in depth order
   void* dsort_ws;      // depth sort
@@ -156,7 +156,8 @@ inline PointLayout point_layout(void* base, uint32_t n) {
   char* p = static_cast<char*>(base);
   size_t o = 0;
   auto take = [&](size_t bytes) { char* q = p ? p + o : nullptr; o += align256(bytes); return q; };
-  L.cnt = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
+  L.rect = reinterpret_cast<uint2*>(take(size_t(n) * 8));
+  L.rect_sorted = reinterpret_cast<uint2*>(take(size_t(n) * 8));
   L.dkey = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
   L.order = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
   L.dsort_bytes = sort_workspace_bytes<uint32_t>(n);
@@ -185,13 +186,15 @@ inline PairLayout pair_layout(void* base, uint32_t cap) {
 template <typename Boxes>
 int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, uint32_t ts,
               uint32_t tw, uint32_t th, const PointLayout& L, uint32_t* d_num_idx) {
+  if (tw > 0xFFFFu || th > 0xFFFFu)  // tile rectangles are packed as 16-bit coordinates
+    return dsplat_set_error(DSPLAT_ERR_ARG, "binning: more than 65535 tiles along an axis");
   if (n == 0) {
     if (d_num_idx) set_u32_one<<<1, 1, 0, s>>>(d_num_idx, 0u);
   } else {
     // per-workgroup sums go to off[] (rewritten by the order phase); with no
     // d_num_idx, K is left to the order phase's scan
     const uint32_t nb = div_up(n, 256);
-    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.cnt,
+    bin_count_kernel<Boxes><<<nb, 256, 0, s>>>(n, boxes, pixxydepth, ts, tw, th, L.rect,
                                                pixxydepth ? L.dkey : nullptr,
                                                d_num_idx ? L.off : nullptr);
     if (d_num_idx) sum_partials_kernel<<<1, 256, 0, s>>>(nb, L.off, d_num_idx);
@@ -201,7 +204,8 @@ int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, u
 }
 
 // Phase 2: stable depth order of the Gaussians (values generated as iota by
-// the first pass; the last pass also gathers the counts into depth order),
+// the first pass; the last pass also gathers the tile rectangles and their
+// areas, the pair counts, into depth order),
 // then the exclusive scan of the counts = each Gaussian's first pair slot;
 // d_total (may be null) receives the scan's total, K.
 inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* d_total = nullptr) {
@@ -212,7 +216,8 @@ inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* 
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
   int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
-                                      L.dsort_bytes, &in_alt, &kalt, &valt, true, L.cnt, L.off);
+                                      L.dsort_bytes, &in_alt, &kalt, &valt, true, L.rect, L.off,
+                                      L.rect_sorted);
   if (rc) return rc;
   if (in_alt)  // 31 key bits take 4 passes: the result is in place
     return dsplat_set_error(DSPLAT_ERR_ARG, "binning: depth order in the alternate buffer");
@@ -229,9 +234,8 @@ int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, u
   if (n == 0) {
     set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
   } else {
-    bin_fill_kernel<Boxes><<<div_up(n, 256), 256, 0, s>>>(n, boxes, L.order, L.off, d_num_idx,
-                                                          capacity, L.misc, ts, tw, th,
-                                                          Q.tilekey, idx2pnt);
+    bin_fill_kernel<<<div_up(n, 256), 256, 0, s>>>(n, L.order, L.rect_sorted, L.off, d_num_idx,
+                                                   capacity, L.misc, tw, Q.tilekey, idx2pnt);
   }
   const uint32_t* tsorted = Q.tilekey;
   if (capacity > 0 && nt > 1) {

@@ -78,6 +78,20 @@ __device__ inline TileRect tile_rect(float x0, float y0, float x1, float y1, uin
   return r;
 }
 
+// A tile rectangle packed into 8 bytes, x0 | y0 << 16 and x1 | y1 << 16
+// (tile coordinates below 65536): what the depth sort carries along with a
+// Gaussian so that the pair expansion reads rectangles in depth order
+// sequentially instead of gathering boxes.
+__device__ inline uint2 pack_rect(const TileRect& r) {
+  return make_uint2(uint32_t(r.x0) | (uint32_t(r.y0) << 16), uint32_t(r.x1) | (uint32_t(r.y1) << 16));
+}
+__device__ inline TileRect unpack_rect(uint2 p) {
+  return TileRect{int(p.x & 0xFFFFu), int(p.y & 0xFFFFu), int(p.x >> 16), int(p.y >> 16)};
+}
+__device__ inline uint32_t rect_area(uint2 p) {
+  return ((p.y & 0xFFFFu) - (p.x & 0xFFFFu)) * ((p.y >> 16) - (p.x >> 16));
+}
+
 // Per-wave LDS slice of wave_expand_pairs.
 struct ExpandLds {
   uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there

@@ -65,7 +65,7 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
                   const float* __restrict__ xyz, const float* __restrict__ scale,
                   const float4* __restrict__ quat, const float* __restrict__ opacity,
                   const float* __restrict__ rgb, float4* __restrict__ rec,
-                  uint32_t* __restrict__ cnt, uint32_t* __restrict__ dkey) {
+                  uint2* __restrict__ rect, uint32_t* __restrict__ dkey) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t v = blockIdx.y;
@@ -90,7 +90,7 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
   r[2] = make_float4(col[2], p.aabb[0], p.aabb[1], p.aabb[2]);
   r[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
   const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
-  cnt[item] = uint32_t(tr.count());
+  rect[item] = pack_rect(tr);
   dkey[item] = depth_key(p.ndcz);
 }
 
@@ -191,9 +191,9 @@ __global__ void mv_clamp_kernel(uint32_t* __restrict__ misc, uint32_t capacity) 
 }
 
 __global__ void __launch_bounds__(256)
-mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
-               const uint32_t* __restrict__ order, const uint32_t* __restrict__ off,
-               const uint32_t* __restrict__ misc, const float4* __restrict__ rec,
+mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views,
+               const uint32_t* __restrict__ order, const uint2* __restrict__ rect_sorted,
+               const uint32_t* __restrict__ off, const uint32_t* __restrict__ misc,
                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ item_of_pair) {
   __shared__ ExpandLds lds[4];
   if (misc[1] == 0) return;  // uniform: overflow, nothing is written
@@ -204,9 +204,7 @@ mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views, u
   if (valid) {
     item = order[j];
     const ViewDev& vd = views[item / n];
-    const float4 q2 = rec[4 * size_t(item) + 2];
-    const float4 q3 = rec[4 * size_t(item) + 3];
-    r = tile_rect(q2.y, q2.z, q2.w, q3.x, ts, vd.tw, vd.th);
+    r = unpack_rect(rect_sorted[j]);  // gathered into depth order by the sort
     o = off[j];
     tbase = vd.tile_off;
     tw = vd.tw;
@@ -411,7 +409,8 @@ struct dsplat_mv_plan {
   size_t mem_bytes = 0;
   ViewDev* d_views = nullptr;
   float4* rec = nullptr;
-  uint32_t *cnt = nullptr, *dkey = nullptr, *order = nullptr;
+  uint2 *rect = nullptr, *rect_sorted = nullptr;  // tile rectangles per item, and in depth order
+  uint32_t *dkey = nullptr, *order = nullptr;
   void* dsort_ws = nullptr;
   size_t dsort_bytes = 0;
   uint32_t *off = nullptr, *scan_ws = nullptr;
@@ -493,6 +492,9 @@ static int fill_views(dsplat_mv_plan* P, uint32_t nview, const dsplat_view_desc*
     toff += vd[v].tw * vd[v].th;
     poff += uint64_t(views[v].img_w) * views[v].img_h;
   }
+  for (uint32_t v = 0; v < nview; ++v)
+    if (vd[v].tw > 0xFFFFu || vd[v].th > 0xFFFFu)  // rectangles are packed as 16-bit tiles
+      return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: more than 65535 tiles along an axis");
   if (toff > P->max_tiles || poff > P->max_pix)
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "multiview: views exceed the plan's pixels/tiles");
   P->host_views = vd;
@@ -530,7 +532,8 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   auto take = [&](size_t b) { size_t r = o; o += a256(b); return r; };
   const size_t o_views = take(sizeof(ViewDev) * max_view);
   const size_t o_rec = take(items * 64);
-  const size_t o_cnt = take(items * 4), o_dkey = take(items * 4), o_order = take(items * 4);
+  const size_t o_rect = take(items * 8), o_rsorted = take(items * 8);
+  const size_t o_dkey = take(items * 4), o_order = take(items * 4);
   P->dsort_bytes = sort_workspace_bytes<uint32_t>(uint32_t(items));
   const size_t o_dsort = take(P->dsort_bytes);
   const size_t o_off = take((items + 1) * 4);
@@ -554,7 +557,8 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   char* b = P->mem;
   P->d_views = reinterpret_cast<ViewDev*>(b + o_views);
   P->rec = reinterpret_cast<float4*>(b + o_rec);
-  P->cnt = reinterpret_cast<uint32_t*>(b + o_cnt);
+  P->rect = reinterpret_cast<uint2*>(b + o_rect);
+  P->rect_sorted = reinterpret_cast<uint2*>(b + o_rsorted);
   P->dkey = reinterpret_cast<uint32_t*>(b + o_dkey);
   P->order = reinterpret_cast<uint32_t*>(b + o_order);
   P->dsort_ws = b + o_dsort;
@@ -761,7 +765,7 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   P->fwd_sh_deg = P->sh_deg;
 #define DSPLAT_MVP(A, B, C)                                                                    \
   mv_project_kernel<A, B, C><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
-                                                col, P->rec, P->cnt, P->dkey)
+                                                col, P->rec, P->rect, P->dkey)
 #define DSPLAT_MVP2(A, B) \
   if (P->sh_k) DSPLAT_MVP(A, B, true); else DSPLAT_MVP(A, B, false)
   if (log_scale && logit) { DSPLAT_MVP2(true, true); }
@@ -773,17 +777,19 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
   // values: list items (iota), generated by the first pass; the last pass
-  // also gathers the tile counts into depth order
+  // also gathers the tile rectangles and their areas (the pair counts) into
+  // depth order
   int rc = radix_sort_pairs<uint32_t>(s, items, nullptr, P->dkey, P->order, 31, P->dsort_ws,
-                                      P->dsort_bytes, &in_alt, &kalt, &valt, true, P->cnt,
-                                      P->off);
+                                      P->dsort_bytes, &in_alt, &kalt, &valt, true, P->rect,
+                                      P->off, P->rect_sorted);
   if (rc) return rc;
   const uint32_t* order = in_alt ? valt : P->order;
   rc = exclusive_scan(s, items, nullptr, P->off, P->off, P->scan_ws, P->misc);
   if (rc) return rc;
   mv_clamp_kernel<<<1, 1, 0, s>>>(P->misc, P->capacity);
-  mv_fill_kernel<<<div_up(items, 256), 256, 0, s>>>(items, n, P->d_views, P->ts, order, P->off,
-                                                     P->misc, P->rec, P->tilekey, P->idx2pnt);
+  mv_fill_kernel<<<div_up(items, 256), 256, 0, s>>>(items, n, P->d_views, order,
+                                                     P->rect_sorted, P->off, P->misc,
+                                                     P->tilekey, P->idx2pnt);
   const uint32_t* tsorted = P->tilekey;
   if (P->capacity > 0 && P->total_tiles > 1) {
     rc = radix_sort_pairs<uint32_t>(s, P->capacity, P->misc + 1, P->tilekey, P->idx2pnt,

@@ -234,8 +234,9 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
   if (threadIdx.x == 0) tot[d] = carry;
 }
 
-// iota: the values are the element indices (vin is not read). gdst (may be
-// null): also writes gdst[pos] = gsrc[value].
+// iota: the values are the element indices (vin is not read). cdst (may be
+// null): also gathers rsrc[value], a packed tile rectangle, writing it to
+// rdst[pos] and its area to cdst[pos].
 template <typename KeyT>
 __global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
@@ -243,7 +244,8 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
                      uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
                      uint32_t nb, const uint32_t* __restrict__ off,
                      const uint32_t* __restrict__ tot, uint32_t nbins,
-                     const uint32_t* __restrict__ gsrc, uint32_t* __restrict__ gdst) {
+                     const uint2* __restrict__ rsrc, uint32_t* __restrict__ cdst,
+                     uint2* __restrict__ rdst) {
   static_assert(kSortBlock == 256, "one thread per digit");
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dpre[256];
@@ -321,11 +323,11 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   // ... and write each digit's run to consecutive global addresses (the
   // gathered values loaded for all rounds before any is stored)
   const uint32_t nvalid = uint32_t(min(uint64_t(kSortTile), n - bstart));
-  uint32_t gv[kSortItems];
+  uint2 gv[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
-    gv[r] = (gdst && i < nvalid) ? gsrc[sval[i]] : 0u;
+    gv[r] = (cdst && i < nvalid) ? rsrc[sval[i]] : make_uint2(0u, 0u);
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
@@ -336,7 +338,10 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
       const uint32_t pos = dbase[d] + (i - dpre[d]);
       kout[pos] = k;
       vout[pos] = sval[i];
-      if (gdst) gdst[pos] = gv[r];
+      if (cdst) {
+        cdst[pos] = rect_area(gv[r]);
+        rdst[pos] = gv[r];
+      }
     }
   }
 }
@@ -350,7 +355,8 @@ template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out,
-                     bool iota_vals, const uint32_t* gather_src, uint32_t* gather_dst) {
+                     bool iota_vals, const uint2* rect_src, uint32_t* cnt_dst,
+                     uint2* rect_dst) {
   *result_in_alt = false;
   if (ws_bytes < sort_workspace_bytes<KeyT>(n_cap))
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
@@ -377,7 +383,7 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
     const bool last = p + 1 == P;
     radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(
         n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt, L.tot,
-        1u << dbits, gather_src, last ? gather_dst : nullptr);
+        1u << dbits, rect_src, last ? cnt_dst : nullptr, last ? rect_dst : nullptr);
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;
@@ -392,10 +398,10 @@ template size_t sort_workspace_bytes<uint32_t>(uint32_t);
 template size_t sort_workspace_bytes<uint64_t>(uint32_t);
 template int radix_sort_pairs<uint32_t>(hipStream_t, uint32_t, const uint32_t*, uint32_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint32_t**,
-                                        uint32_t**, bool, const uint32_t*, uint32_t*);
+                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*);
 template int radix_sort_pairs<uint64_t>(hipStream_t, uint32_t, const uint32_t*, uint64_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint64_t**,
-                                        uint32_t**, bool, const uint32_t*, uint32_t*);
+                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*);
 
 // ------------------------------------------------------------------ fill ----
 __global__ void fill_f32_kernel(uint32_t n, float* __restrict__ buf, float v) {

@@ -29,8 +29,8 @@ template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt,
-                     bool iota_vals = false, const uint32_t* gather_src = nullptr,
-                     uint32_t* gather_dst = nullptr);
+                     bool iota_vals = false, const uint2* rect_src = nullptr,
+                     uint32_t* cnt_dst = nullptr, uint2* rect_dst = nullptr);
 
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 
