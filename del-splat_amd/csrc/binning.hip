@@ -99,16 +99,13 @@ bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
   wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair);
 }
 
-// tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T].
+// tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T]
+// (tile_ranges4: four positions per thread).
 __global__ void __launch_bounds__(256)
 tile_ranges_kernel(const uint32_t* __restrict__ n_eff, uint32_t capacity, uint32_t num_tile,
                    const uint32_t* __restrict__ tile_sorted, uint32_t* __restrict__ tile2idx) {
-  const uint32_t k = *n_eff;
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx > k || idx > capacity) return;
-  const uint32_t lo = idx == 0 ? 0u : tile_sorted[idx - 1] + 1u;
-  const uint32_t hi = idx == k ? num_tile : tile_sorted[idx];
-  for (uint32_t t = lo; t <= hi; ++t) tile2idx[t] = idx;
+  const uint32_t k = min(*n_eff, capacity);
+  tile_ranges4(blockIdx.x * blockDim.x + threadIdx.x, k, num_tile, tile_sorted, tile2idx);
 }
 
 __global__ void __launch_bounds__(256)
@@ -250,8 +247,10 @@ int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, u
       tsorted = kalt;
     }
   }
-  tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 256), 256, 0, s>>>(L.misc, capacity, nt,
-                                                                         tsorted, tile2idx);
+  if (reinterpret_cast<uintptr_t>(tsorted) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "binning: sorted tile ids not 16-byte aligned");
+  tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 1024), 256, 0, s>>>(L.misc, capacity, nt,
+                                                                          tsorted, tile2idx);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }

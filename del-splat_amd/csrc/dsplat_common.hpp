@@ -92,6 +92,34 @@ __device__ inline uint32_t rect_area(uint2 p) {
   return ((p.y & 0xFFFFu) - (p.x & 0xFFFFu)) * ((p.y >> 16) - (p.x >> 16));
 }
 
+// tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T],
+// from k sorted tile ids (16-byte aligned), for the four positions
+// [4q, 4q + 4) of thread q (the position k stands for tile T): one 16-byte
+// load and the previous id, and a store for every tile a boundary passes.
+__device__ inline void tile_ranges4(uint32_t q, uint32_t k, uint32_t num_tile,
+                                    const uint32_t* __restrict__ tile_sorted,
+                                    uint32_t* __restrict__ tile2idx) {
+  const uint32_t i0 = 4u * q;
+  if (i0 > k) return;
+  uint32_t t[4];
+  if (i0 + 4u <= k) {
+    const uint4 v = reinterpret_cast<const uint4*>(tile_sorted)[q];
+    t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) t[e] = i0 + e < k ? tile_sorted[i0 + e] : num_tile;
+  }
+  uint32_t lo = i0 == 0 ? 0u : tile_sorted[i0 - 1] + 1u;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t idx = i0 + uint32_t(e);
+    if (idx > k) break;
+    const uint32_t hi = idx == k ? num_tile : t[e];
+    for (uint32_t tt = lo; tt <= hi; ++tt) tile2idx[tt] = idx;
+    lo = hi + 1u;
+  }
+}
+
 // Per-wave LDS slice of wave_expand_pairs.
 struct ExpandLds {
   uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there

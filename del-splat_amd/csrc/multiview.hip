@@ -216,12 +216,8 @@ mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views,
 __global__ void __launch_bounds__(256)
 mv_ranges_kernel(const uint32_t* __restrict__ misc, uint32_t capacity, uint32_t num_tile,
                  const uint32_t* __restrict__ tile_sorted, uint32_t* __restrict__ tile2idx) {
-  const uint32_t k = misc[1];
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx > k || idx > capacity) return;
-  const uint32_t lo = idx == 0 ? 0u : tile_sorted[idx - 1] + 1u;
-  const uint32_t hi = idx == k ? num_tile : tile_sorted[idx];
-  for (uint32_t t = lo; t <= hi; ++t) tile2idx[t] = idx;
+  tile_ranges4(blockIdx.x * blockDim.x + threadIdx.x, min(misc[1], capacity), num_tile,
+               tile_sorted, tile2idx);
 }
 
 __global__ void __launch_bounds__(256)
@@ -801,7 +797,9 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
       tsorted = kalt;
     }
   }
-  mv_ranges_kernel<<<div_up(uint64_t(P->capacity) + 1, 256), 256, 0, s>>>(
+  if (reinterpret_cast<uintptr_t>(tsorted) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: sorted tile ids not 16-byte aligned");
+  mv_ranges_kernel<<<div_up(uint64_t(P->capacity) + 1, 1024), 256, 0, s>>>(
       P->misc, P->capacity, P->total_tiles, tsorted, P->tile2idx);
   const uint32_t ts = P->ts, th = P->threads;
   if (ts == 8) launch_mv_fwd<8, 64>(P, s, out_rgb, out_alpha);
