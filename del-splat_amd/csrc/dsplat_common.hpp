@@ -120,6 +120,13 @@ __device__ inline void tile_ranges4(uint32_t q, uint32_t k, uint32_t num_tile,
   }
 }
 
+// The calling wave's LDS writes are visible to its following LDS reads.
+__device__ inline void wave_lds_fence() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-wave LDS slice of wave_expand_pairs.
 struct ExpandLds {
   uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there

@@ -66,32 +66,47 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
                   const float4* __restrict__ quat, const float* __restrict__ opacity,
                   const float* __restrict__ rgb, float4* __restrict__ rec,
                   uint2* __restrict__ rect, uint32_t* __restrict__ dkey) {
+  // the records leave through LDS as contiguous 1 KB stores per wave (the
+  // wave's 64 items are consecutive), as in gauss_project_kernel
+  __shared__ float4 srec[4 * 256];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
+  const uint32_t g0 = blockIdx.x * blockDim.x + wbase;
+  if (g0 >= n) return;  // wave-uniform
   const uint32_t v = blockIdx.y;
   const ViewDev& vd = views[v];
-  const float4 q = quat[i];
-  float s0 = scale[3 * i], s1 = scale[3 * i + 1], s2 = scale[3 * i + 2];
-  if (kLogScale) {
-    s0 = expf(s0);
-    s1 = expf(s1);
-    s2 = expf(s2);
+  if (i < n) {
+    const float4 q = quat[i];
+    float s0 = scale[3 * i], s1 = scale[3 * i + 1], s2 = scale[3 * i + 2];
+    if (kLogScale) {
+      s0 = expf(s0);
+      s1 = expf(s1);
+      s2 = expf(s2);
+    }
+    ProjState p;
+    project_state(vd.m, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q.x, q.y, q.z, q.w, s0, s1,
+                  s2, vd.w, vd.h, p);
+    float op = opacity[i];
+    if (kLogitOp) op = sigmoidf_(op);
+    const size_t item = size_t(v) * n + i;
+    const float* col = rgb + 3 * (kRgbItem ? item : size_t(i));
+    float4* mine = srec + 4 * threadIdx.x;
+    mine[0] = make_float4(p.pix[0], p.pix[1], p.conic[0], p.conic[1]);
+    mine[1] = make_float4(p.conic[2], op, col[0], col[1]);
+    mine[2] = make_float4(col[2], p.aabb[0], p.aabb[1], p.aabb[2]);
+    mine[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
+    const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
+    rect[item] = pack_rect(tr);
+    dkey[item] = depth_key(p.ndcz);
   }
-  ProjState p;
-  project_state(vd.m, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q.x, q.y, q.z, q.w, s0, s1,
-                s2, vd.w, vd.h, p);
-  float op = opacity[i];
-  if (kLogitOp) op = sigmoidf_(op);
-  const size_t item = size_t(v) * n + i;
-  const float* col = rgb + 3 * (kRgbItem ? item : size_t(i));
-  float4* r = rec + 4 * item;
-  r[0] = make_float4(p.pix[0], p.pix[1], p.conic[0], p.conic[1]);
-  r[1] = make_float4(p.conic[2], op, col[0], col[1]);
-  r[2] = make_float4(col[2], p.aabb[0], p.aabb[1], p.aabb[2]);
-  r[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
-  const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
-  rect[item] = pack_rect(tr);
-  dkey[item] = depth_key(p.ndcz);
+  wave_lds_fence();  // every lane of the wave is past its LDS writes
+  const uint32_t nrec4 = 4u * min(64u, n - g0);
+  float4* out = rec + 4 * (size_t(v) * n + g0);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t k = uint32_t(c) * 64u + lane;
+    if (k < nrec4) out[k] = srec[4 * wbase + k];
+  }
 }
 
 // ----------------------------------------------------------- SH colour ---
