@@ -100,6 +100,9 @@ struct GroupMap {
   static constexpr int RXN = TS / RS;
   static constexpr int G = RXN * RXN / W;
   static constexpr int PPT = G * Q;
+  // 8 x 8 regions, one pixel per lane: a list entry's box test for a region
+  // is a 64-bit lane mask (region_lane_mask) formed in scalar registers
+  static constexpr bool kLaneMask = RS == 8;
   static_assert(TS % RS == 0 && G >= 1 && G * W == RXN * RXN, "unsupported tile / workgroup shape");
   __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
   __device__ static uint32_t rx0(int g) { return ((wave() * G + g) % RXN) * RS; }
@@ -139,6 +142,137 @@ __device__ inline uint32_t region_mask(uint32_t tx, uint32_t ty, float bx0, floa
   return m;
 }
 
+// The columns c in [0, TS) of a tile starting at pixel X0 whose centre
+// p_c = float(X0 + c) + 0.5f satisfies b0 <= p_c <= b1, as bits (the same
+// float comparisons as the per-pixel box test, answered for a whole row of
+// the tile at once). The set is an interval [lo, hi]: lo is the first c
+// with p_c >= b0, hi the last with p_c <= b1. Candidates from ceil / floor
+// of the offsets are within one of the answer (the offsets are exact or
+// within rounding of it wherever the clamp does not decide), and one
+// comparison step each way makes them exact. NaN or b0 > b1: no centre.
+template <int TS>
+__host__ __device__ inline uint32_t axis_mask(uint32_t X0, float b0, float b1) {
+  static_assert(TS <= 32, "one bit per column");
+  if (!(b0 <= b1)) return 0u;
+  const float base = float(X0) + 0.5f;
+  int lo = int(ceilf(fminf(fmaxf(b0 - base, -1.f), float(TS))));
+  lo = lo < 0 ? 0 : lo;
+  if (lo > 0 && float(X0 + uint32_t(lo) - 1u) + 0.5f >= b0) --lo;
+  if (lo < TS && !(float(X0 + uint32_t(lo)) + 0.5f >= b0)) ++lo;
+  int hi = int(floorf(fminf(fmaxf(b1 - base, -1.f), float(TS))));
+  hi = hi > TS - 1 ? TS - 1 : hi;
+  if (hi < TS - 1 && float(X0 + uint32_t(hi + 1)) + 0.5f <= b1) ++hi;
+  if (hi >= 0 && !(float(X0 + uint32_t(hi)) + 0.5f <= b1)) --hi;
+  if (lo > hi) return 0u;
+  return ((2u << uint32_t(hi - lo)) - 1u) << uint32_t(lo);
+}
+
+// Lanes of the 8 x 8 region at (rx0, ry0) (lane = 8 row + column, GroupMap's
+// layout with one pixel per lane) whose pixel lies in the box, from the
+// tile's column and row masks: the region's column byte repeated in every
+// row whose row bit is set. Row expansion: bit i of a nibble times
+// 0x00204081 lands on bit 8i (the four shifted copies do not overlap), and
+// times 0xFF fills its byte.
+__host__ __device__ inline uint64_t region_lane_mask(uint32_t xm, uint32_t ym, uint32_t rx0,
+                                                     uint32_t ry0) {
+  const uint32_t xb = (xm >> rx0) & 0xFFu, yb = (ym >> ry0) & 0xFFu;
+  const uint32_t xr = xb * 0x01010101u;
+  const uint32_t ylo = (((yb & 15u) * 0x00204081u) & 0x01010101u) * 0xFFu;
+  const uint32_t yhi = (((yb >> 4) * 0x00204081u) & 0x01010101u) * 0xFFu;
+  return (uint64_t(xr & yhi) << 32) | uint64_t(xr & ylo);
+}
+
+// Per-entry masks in LDS. Region form: one word per entry (region_mask).
+// Lane-mask form: the 64-bit lane mask of every region, stored [region][B]
+// (consecutive entries in consecutive slots: the staging threads write
+// without bank conflicts, and a wave reads only its own regions).
+template <int TS, int NT, int B>
+struct EntryMasks {
+  using GM = GroupMap<TS, NT>;
+  static constexpr int NR = GM::RXN * GM::RXN;
+  uint32_t region[GM::kLaneMask ? 1 : B];
+  uint64_t lanes[GM::kLaneMask ? NR * B : 1];
+
+  // By the staging thread of entry `slot`, for the box (bx0, by0, bx1, by1).
+  __device__ void stage(uint32_t slot, uint32_t tx, uint32_t ty, float bx0, float by0, float bx1,
+                        float by1) {
+    if constexpr (GM::kLaneMask) {
+      const uint32_t xm = axis_mask<TS>(tx * TS, bx0, bx1), ym = axis_mask<TS>(ty * TS, by0, by1);
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        lanes[r * B + slot] =
+            region_lane_mask(xm, ym, uint32_t(r % GM::RXN) * 8u, uint32_t(r / GM::RXN) * 8u);
+    } else {
+      region[slot] = region_mask<TS, NT>(tx, ty, bx0, by0, bx1, by1);
+    }
+  }
+};
+
+// Scalar per-entry hit data of a wave: bit g of `hits` when group g can take
+// the entry, and with lane masks each group's lanes inside the box.
+template <int TS, int NT>
+struct EntryHits {
+  uint32_t hits;
+  uint64_t lanes[GroupMap<TS, NT>::kLaneMask ? GroupMap<TS, NT>::G : 1];
+};
+
+// Entry j's hit data for the wave whose first group is region `first`
+// (wave * G, formed once outside the entry loops).
+template <int TS, int NT, int B>
+__device__ __forceinline__ EntryHits<TS, NT> entry_hits(const EntryMasks<TS, NT, B>& m,
+                                                        uint32_t j, uint32_t first) {
+  using GM = GroupMap<TS, NT>;
+  EntryHits<TS, NT> h;
+  if constexpr (GM::kLaneMask) {
+    h.hits = 0;
+#pragma unroll
+    for (int g = 0; g < GM::G; ++g) {
+      const uint64_t v = m.lanes[(first + uint32_t(g)) * B + j];
+      const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+      const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+      h.lanes[g] = (uint64_t(hi) << 32) | lo;
+      h.hits |= uint32_t(h.lanes[g] != 0) << g;
+    }
+  } else {
+    h.hits = (__builtin_amdgcn_readfirstlane(m.region[j]) >> first) & ((1u << GM::G) - 1u);
+    h.lanes[0] = 0;
+  }
+  return h;
+}
+
+// Lane-mask form (one pixel per lane): the lanes of group g that take the
+// entry are the walking lanes (a ballot) inside the box (the group's lane
+// mask), combined in scalar registers. Returns false when there are none
+// (wave-uniform).
+template <int TS, int NT>
+__device__ __forceinline__ bool lanes_take(const EntryHits<TS, NT>& h, int g, uint64_t walking,
+                                           bool (&inm)[GroupMap<TS, NT>::Q]) {
+  const uint64_t act = walking & h.lanes[g];
+  inm[0] = __builtin_amdgcn_inverse_ballot_w64(act);
+  return act != 0;
+}
+
+// Box-test form: pixel slot k of group g takes the entry when walking[k]
+// and its centre lies in the box q2 / q3 (inclusive, aabb2::is_include_point;
+// non-short-circuit &: plain compares and mask ands, no branches).
+template <int TS, int NT>
+__device__ __forceinline__ bool box_take(int g, const bool (&walking)[GroupMap<TS, NT>::PPT],
+                                         const float (&px)[GroupMap<TS, NT>::PPT],
+                                         const float (&py)[GroupMap<TS, NT>::PPT],
+                                         const float4& q2, const float4& q3,
+                                         bool (&inm)[GroupMap<TS, NT>::Q]) {
+  using GM = GroupMap<TS, NT>;
+  bool any = false;
+#pragma unroll
+  for (int qq = 0; qq < GM::Q; ++qq) {
+    const int k = g * GM::Q + qq;
+    inm[qq] = walking[k] & (px[k] >= q2.y) & (px[k] <= q2.w) & (py[k] >= q2.z) &
+              (py[k] <= q3.x);
+    any = any || inm[qq];
+  }
+  return __ballot(any) != 0;
+}
+
 // Forward of one tile. Image arrays are the view's base pointers (pixel
 // index iy * img_w + ix); alpha / nwalk / tlast may be null. The entry loop is
 // outermost so one LDS read of a record serves all PPT pixels of a thread.
@@ -160,9 +294,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
   // typical densities, so larger batches would load records nobody reads
   constexpr int B = NT < 128 ? NT : 128;
   __shared__ float4 s0[B], s1[B], s2[B], s3[B];
-  __shared__ uint32_t smask[B];
+  __shared__ EntryMasks<TS, NT, B> smask;
   const uint32_t num = end - start;
-  const uint32_t gshift = GM::wave() * uint32_t(G), gbits = (1u << G) - 1u;
+  const uint32_t gfirst = GM::wave() * uint32_t(G);
   // A pixel is live while !(alpha_sum >= 0.999f) (NaN stays live, as in the
   // reference's test); pixels outside the image start at alpha_sum = 1.
   float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
@@ -188,7 +322,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
       s3[threadIdx.x] = q3;
-      smask[threadIdx.x] = region_mask<TS, NT>(tx, ty, q2.y, q2.z, q2.w, q3.x);
+      smask.stage(threadIdx.x, tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     __syncthreads();
     const uint32_t cnt = min(uint32_t(B), num - b);
@@ -197,16 +331,19 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
     for (uint32_t j = 0; j < cnt; ++j) {
       // groups of this wave that the box reaches (scalar)
-      const uint32_t hits = (__builtin_amdgcn_readfirstlane(smask[j]) >> gshift) & gbits;
+      const EntryHits<TS, NT> eh = entry_hits(smask, j, gfirst);
+      const uint32_t hits = eh.hits;
       if (hits == 0) continue;
       bool live[PPT];
-      bool anylive = false;
+      uint64_t live_lanes[PPT];  // ballots of live (scalar)
+      uint64_t anylive = 0;
 #pragma unroll
       for (int k = 0; k < PPT; ++k) {
         live[k] = !(asum[k] >= 0.999f);
-        anylive = anylive || live[k];
+        live_lanes[k] = __ballot(!(asum[k] >= 0.999f));
+        anylive |= live_lanes[k];
       }
-      if (__ballot(anylive) == 0) break;  // every pixel of the wave is done
+      if (anylive == 0) break;  // every pixel of the wave is done
       const float4 q2 = s2[j];
       const float4 q3 = s3[j];
       const float4 q0 = s0[j];
@@ -216,17 +353,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       for (int g = 0; g < G; ++g) {
         if (!((hits >> g) & 1u)) continue;  // wave-uniform
         bool inm[Q];
-        bool any = false;
-#pragma unroll
-        for (int qq = 0; qq < Q; ++qq) {
-          const int k = g * Q + qq;
-          // inclusive box test (aabb2::is_include_point)
-          // non-short-circuit &: plain compares and mask ands, no branches
-          inm[qq] = live[k] & (px[k] >= q2.y) & (px[k] <= q2.w) & (py[k] >= q2.z) &
-                    (py[k] <= q3.x);
-          any = any || inm[qq];
-        }
-        if (__ballot(any) == 0) continue;  // no live pixel of the group is inside the box
+        // no live pixel of the group is inside the box
+        if (GM::kLaneMask ? !lanes_take<TS, NT>(eh, g, live_lanes[g], inm)
+                          : !box_take<TS, NT>(g, live, px, py, q2, q3, inm))
+          continue;
 #pragma unroll
         for (int qq = 0; qq < Q; ++qq) {
           const int k = g * Q + qq;
@@ -410,14 +540,14 @@ struct GradSink {
 // partial-sum slot per wave (waves combine by plain stores); with more, one
 // slot that the waves add into with LDS float atomics (per-wave slots would
 // take so much LDS that fewer workgroups fit on a CU).
-template <int NT>
+template <int TS, int NT>
 struct BwdLds {
   static constexpr int B = NT < 128 ? NT : 128;
   static constexpr int S = NT / 64 <= 2 ? NT / 64 : 1;
   float4 s0[B], s1[B], s2[B], s3[B];
   uint32_t spid[B];
   float acc[S][kGrad][B];
-  uint32_t smask[B];  // region_mask of each staged entry
+  EntryMasks<TS, NT, B> smask;
   uint32_t smax;
 };
 
@@ -436,7 +566,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float* __restrict__ target,
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out,
-                                                BwdLds<NT>& L) {
+                                                BwdLds<TS, NT>& L) {
 #pragma clang fp contract(off)
   using GM = GroupMap<TS, NT>;
   constexpr int PPT = GM::PPT, G = GM::G, Q = GM::Q, W = GM::W;
@@ -445,11 +575,11 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   float4* s2 = L.s2;
   float4* s3 = L.s3;
   uint32_t* spid = L.spid;
-  uint32_t* smask = L.smask;
+  EntryMasks<TS, NT, BwdLds<TS, NT>::B>& smask = L.smask;
   uint32_t& smax = L.smax;
   const uint32_t wv = GM::wave();
-  const uint32_t gshift = wv * uint32_t(G), gbits = (1u << G) - 1u;
-  constexpr int B = BwdLds<NT>::B, S = BwdLds<NT>::S;
+  const uint32_t gfirst = wv * uint32_t(G);
+  constexpr int B = BwdLds<TS, NT>::B, S = BwdLds<TS, NT>::S;
   float(&acc)[kGrad][B] = L.acc[S == 1 ? 0 : wv];
   const int lane = threadIdx.x & 63;
   float px[PPT], py[PPT], tr[PPT], g0[PPT], g1[PPT], g2[PPT], ga[PPT];
@@ -504,13 +634,16 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const uint32_t it = hi - 1 - threadIdx.x;
       const uint32_t p = idx2pnt[end - 1 - it];
       const float4* r = splat2 + 4 * size_t(p);
-      const float4 q2 = r[2], q3 = r[3];
-      s0[threadIdx.x] = r[0];
-      s1[threadIdx.x] = r[1];
+      const float4 q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
+      // the conic scaled for exp2 where the walk reads it, the raw conic
+      // in the record's free slots for the flush
+      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
+      s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
-      s3[threadIdx.x] = q3;
+      s3[threadIdx.x] = make_float4(q3.x, q0.z, q0.w, q1.x);
       spid[threadIdx.x] = p;
-      smask[threadIdx.x] = region_mask<TS, NT>(tx, ty, q2.y, q2.z, q2.w, q3.x);
+      smask.stage(threadIdx.x, tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     // partial-sum slots start at zero (entries a wave skips stay zero)
     if (S > 1) {
@@ -527,11 +660,12 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const uint32_t it = hi - 1 - j;
       if (it >= wmax) continue;  // beyond every walk of the wave
       // groups of this wave that the box reaches (scalar)
-      const uint32_t hits = (__builtin_amdgcn_readfirstlane(smask[j]) >> gshift) & gbits;
+      const EntryHits<TS, NT> eh = entry_hits(smask, j, gfirst);
+      const uint32_t hits = eh.hits;
       if (hits == 0) continue;
       const float4 q2 = s2[j], q3 = s3[j];
       const float4 q0 = s0[j], q1 = s1[j];
-      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
       // -0 so that the first group's terms enter without an add (x + -0 = x)
       float v[kGrad];
 #pragma unroll
@@ -540,16 +674,14 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         if (!((hits >> g) & 1u)) continue;  // wave-uniform
-        bool inm[Q];
-        bool any = false;
+        bool walking[PPT];
 #pragma unroll
-        for (int qq = 0; qq < Q; ++qq) {
-          const int k = g * Q + qq;
-          inm[qq] = (it < nw[k]) & (px[k] >= q2.y) & (px[k] <= q2.w) & (py[k] >= q2.z) &
-                    (py[k] <= q3.x);
-          any = any || inm[qq];
-        }
-        if (__ballot(any) == 0) continue;  // no pixel of the group takes this entry
+        for (int k = 0; k < PPT; ++k) walking[k] = it < nw[k];
+        bool inm[Q];
+        // no pixel of the group takes this entry
+        if (GM::kLaneMask ? !lanes_take<TS, NT>(eh, g, __ballot(it < nw[g]), inm)
+                          : !box_take<TS, NT>(g, walking, px, py, q2, q3, inm))
+          continue;
         touched = true;
         // Branch-free per pixel: a splat that misses the pixel gets gk = e = 0,
         // which adds exact zeros and keeps T (T * rcp(1) = T).
@@ -615,7 +747,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       }
       const uint32_t p = spid[t];
       const uint32_t g = p - gauss_base;
-      const float ca = s0[t].z, cb = s0[t].w, cc = s1[t].x;
+      const float ca = s3[t].y, cb = s3[t].z, cc = s3[t].w;
       // sum[0..4] are moments of -2 dL/dq. d q / d mean = -2 C d, so
       // dL/dmean = -2 C (S_x, S_y) = C (sum[0], sum[1]) (scaling by powers
       // of two is exact: the same bits as scaling every term)
@@ -652,7 +784,7 @@ __device__ __forceinline__ void raster_tile_bwd(uint32_t tx, uint32_t ty, uint32
                                                 const float* __restrict__ target,
                                                 const float* __restrict__ d_alpha,
                                                 uint32_t gauss_base, const GradSink& out) {
-  __shared__ BwdLds<NT> lds;
+  __shared__ BwdLds<TS, NT> lds;
   if (d_alpha)
     raster_tile_bwd_impl<TS, NT, true>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
                                        pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
