@@ -489,3 +489,89 @@ def test_headline_frame_matches_oracle(oracle):
         err = max_rel(g.cpu().numpy().reshape(-1), np.asarray(o).reshape(-1))
         print(f"{name}: max rel err {err:.2e}")
         assert err <= 2e-3, (name, err)
+
+
+def _crafted_boxes(n, shape, seed):
+    """Splats whose boxes put their edges exactly on pixel centres, one ulp
+    either side of them, at random positions, or at NaN / +-inf / +-1e30, some
+    inverted; every tile lists every splat (in depth order), so the box test
+    alone decides which pixels a splat reaches."""
+    w, h = shape
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(-10, w + 10, n)
+    cy = rng.uniform(-10, h + 10, n)
+    sig = rng.uniform(2.0, 12.0, (n, 2))
+    th = rng.uniform(0, np.pi, n)
+    c, s_ = np.cos(th), np.sin(th)
+    a = c * c / sig[:, 0] ** 2 + s_ * s_ / sig[:, 1] ** 2
+    b = c * s_ * (1 / sig[:, 0] ** 2 - 1 / sig[:, 1] ** 2)
+    cc = s_ * s_ / sig[:, 0] ** 2 + c * c / sig[:, 1] ** 2
+    r = 3.0 * sig.max(axis=1)
+    box = np.stack([cx - r, cy - r, cx + r, cy + r], 1).astype(np.float32)
+    special = np.float32([np.nan, np.inf, -np.inf, 1e30, -1e30])
+    for i in range(n):
+        for e in range(4):
+            mode = rng.integers(0, 7)
+            v = box[i, e]
+            if mode in (1, 2, 3):
+                v = np.float32(np.floor(v) + 0.5)  # a pixel centre
+                if mode == 2:
+                    v = np.nextafter(v, np.float32(np.inf))
+                elif mode == 3:
+                    v = np.nextafter(v, np.float32(-np.inf))
+            elif mode == 4 and rng.random() < 0.15:
+                v = special[rng.integers(0, len(special))]
+            box[i, e] = v
+        if rng.random() < 0.05:
+            box[i, [0, 2]] = box[i, [2, 0]]  # inverted
+    pxd = np.stack([cx, cy, rng.uniform(0.5, 5.0, n)], 1).astype(np.float32)
+    conic = np.stack([a, b, cc], 1).astype(np.float32)
+    op = rng.uniform(0.2, 0.9, n).astype(np.float32)
+    rgb = rng.random((n, 3)).astype(np.float32)
+    splat2 = np.zeros((n, 16), np.float32)
+    splat2[:, 0:2] = pxd[:, :2]
+    splat2[:, 2:5] = conic
+    splat2[:, 5] = op
+    splat2[:, 6:9] = rgb
+    splat2[:, 9:13] = box
+    return pxd, conic, box, op, rgb, splat2
+
+
+@pytest.mark.parametrize("ts,threads", [(8, 64), (16, 64), (16, 128), (16, 256), (32, 256),
+                                        (32, 512), (32, 1024)])
+def test_box_edges_on_pixel_centres(oracle, ts, threads):
+    """The inclusive box test at its edges (the raster forms it per tile row /
+    column and per 8x8 pixel region, or per pixel, by workgroup shape):
+    forward pixels and the adjoint against the oracle's per-pixel float
+    comparisons on boxes crafted to sit on, and one ulp off, pixel centres."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (100, 70)
+    n = 300
+    pxd, conic, box, op, rgb, splat2 = _crafted_boxes(n, shape, 7 + ts + threads)
+    ntile = ((shape[0] + ts - 1) // ts) * ((shape[1] + ts - 1) // ts)
+    order = np.argsort(pxd[:, 2], kind="stable").astype(np.uint32)
+    t2i = (np.arange(ntile + 1) * n).astype(np.uint32)
+    i2p = np.tile(order, ntile)
+    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(shape, pxd, conic, box, op, rgb,
+                                                              ts, t2i, i2p)
+    g_t2i, g_i2p = _t(t2i.view(np.int32)), _t(i2p.view(np.int32))
+    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, _t(splat2), g_t2i, g_i2p, threads)
+    _check_pixels(img.cpu().numpy(), alpha.cpu().numpy(), nwalk.cpu().numpy().view(np.uint32),
+                  o_img, o_alpha, o_nwalk)
+    d_img = np.random.default_rng(ts).standard_normal((shape[1], shape[0], 3)).astype(np.float32)
+    g = SG.rasterize_bwd(shape, ts, _t(splat2), g_t2i, g_i2p, nwalk, tlast, _t(d_img), None,
+                         threads)
+    o = oracle.gauss_rasterize_bwd(shape, pxd, conic, box, op, rgb, ts, t2i, i2p,
+                                   nwalk.cpu().numpy().view(np.uint32), tlast.cpu().numpy(),
+                                   d_img)
+    for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
+        err = max_rel(a.cpu().numpy(), b)
+        print(f"ts={ts} threads={threads} {name}: max rel err {err:.2e}", flush=True)
+        assert err <= 2e-3, (name, err)
+    # the crafted edges do decide pixels: moving every edge one ulp outward
+    # changes the image
+    wide = splat2.copy()
+    for e, d in ((9, -np.inf), (10, -np.inf), (11, np.inf), (12, np.inf)):
+        wide[:, e] = np.nextafter(wide[:, e], np.float32(d))
+    img2 = SG.rasterize_fwd(shape, ts, _t(wide), g_t2i, g_i2p, threads)[0]
+    assert (img2 != img).any()
