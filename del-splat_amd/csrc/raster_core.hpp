@@ -325,52 +325,60 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       smask.stage(threadIdx.x, tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     __syncthreads();
-    const uint32_t cnt = min(uint32_t(B), num - b);
+    // scalar (a min formed as v_med3 would put the loop bound in a VGPR)
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(min(uint32_t(B), num - b));
     // Branch-free per pixel: a splat that misses the pixel (or a finished
     // pixel) gets e = 0, which leaves every accumulator bit-identical
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
+    // No continue / break in this loop: jumps out of a scope with locals
+    // were compiled into a per-entry exit-state switch in scalar code.
     for (uint32_t j = 0; j < cnt; ++j) {
       // groups of this wave that the box reaches (scalar)
       const EntryHits<TS, NT> eh = entry_hits(smask, j, gfirst);
       const uint32_t hits = eh.hits;
-      if (hits == 0) continue;
-      bool live[PPT];
-      uint64_t live_lanes[PPT];  // ballots of live (scalar)
-      uint64_t anylive = 0;
+      if (hits != 0) {
+        bool live[PPT];
+        uint64_t live_lanes[PPT];  // ballots of live (scalar)
+        uint64_t anylive = 0;
 #pragma unroll
-      for (int k = 0; k < PPT; ++k) {
-        live[k] = !(asum[k] >= 0.999f);
-        live_lanes[k] = __ballot(!(asum[k] >= 0.999f));
-        anylive |= live_lanes[k];
-      }
-      if (anylive == 0) break;  // every pixel of the wave is done
-      const float4 q2 = s2[j];
-      const float4 q3 = s3[j];
-      const float4 q0 = s0[j];
-      const float4 q1 = s1[j];
-      const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
+        for (int k = 0; k < PPT; ++k) {
+          live[k] = !(asum[k] >= 0.999f);
+          live_lanes[k] = __ballot(!(asum[k] >= 0.999f));
+          anylive |= live_lanes[k];
+        }
+        if (anylive == 0) {
+          j = cnt;  // every pixel of the wave is done: leave the batch
+        } else {
+          const float4 q2 = s2[j];
+          const float4 q3 = s3[j];
+          const float4 q0 = s0[j];
+          const float4 q1 = s1[j];
+          const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (!((hits >> g) & 1u)) continue;  // wave-uniform
-        bool inm[Q];
-        // no live pixel of the group is inside the box
-        if (GM::kLaneMask ? !lanes_take<TS, NT>(eh, g, live_lanes[g], inm)
-                          : !box_take<TS, NT>(g, live, px, py, q2, q3, inm))
-          continue;
+          for (int g = 0; g < G; ++g) {
+            bool inm[Q];
+            // wave-uniform: the box reaches the group and a live pixel of
+            // it lies inside
+            if (((hits >> g) & 1u) &&
+                (GM::kLaneMask ? lanes_take<TS, NT>(eh, g, live_lanes[g], inm)
+                               : box_take<TS, NT>(g, live, px, py, q2, q3, inm))) {
 #pragma unroll
-        for (int qq = 0; qq < Q; ++qq) {
-          const int k = g * Q + qq;
-          const bool in = inm[qq];
-          const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-          const float e = splat_weight(in, ke, dx, dy) * q1.y;
-          const float eo = tr[k] * e;
-          cr[k] = fmaf(q1.z, eo, cr[k]);
-          cg[k] = fmaf(q1.w, eo, cg[k]);
-          cbl[k] = fmaf(q2.x, eo, cbl[k]);
-          tl[k] = in ? tr[k] : tl[k];
-          walked[k] = in ? b + j + 1 : walked[k];
-          tr[k] = tr[k] * (1.f - e);
-          asum[k] = asum[k] + eo;
+              for (int qq = 0; qq < Q; ++qq) {
+                const int k = g * Q + qq;
+                const bool in = inm[qq];
+                const float dx = px[k] - q0.x, dy = py[k] - q0.y;
+                const float e = splat_weight(in, ke, dx, dy) * q1.y;
+                const float eo = tr[k] * e;
+                cr[k] = fmaf(q1.z, eo, cr[k]);
+                cg[k] = fmaf(q1.w, eo, cg[k]);
+                cbl[k] = fmaf(q2.x, eo, cbl[k]);
+                tl[k] = in ? tr[k] : tl[k];
+                walked[k] = in ? b + j + 1 : walked[k];
+                tr[k] = tr[k] * (1.f - e);
+                asum[k] = asum[k] + eo;
+              }
+            }
+          }
         }
       }
     }
@@ -626,10 +634,10 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   __syncthreads();
   if (lane == 0) atomicMax(&smax, wmax);
   __syncthreads();
-  const uint32_t maxw = smax;
+  const uint32_t maxw = __builtin_amdgcn_readfirstlane(smax);  // scalar loop bounds
   for (uint32_t hi = maxw; hi > 0;) {
     const uint32_t lo = hi > uint32_t(B) ? hi - B : 0u;
-    const uint32_t cnt = hi - lo;
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(hi - lo);  // scalar loop bound
     if (threadIdx.x < cnt) {
       const uint32_t it = hi - 1 - threadIdx.x;
       const uint32_t p = idx2pnt[end - 1 - it];
