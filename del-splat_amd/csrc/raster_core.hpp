@@ -77,10 +77,11 @@ __device__ __forceinline__ ConicE2 conic_e2(float a, float b, float c) {
 }
 
 // exp(-q/2) where in, else 0 (2^-inf = 0: a select on the argument keeps
-// the exp unpredicated).
+// the exp unpredicated). The exponent as dx (b2' dy + a' dx) + (c' dy) dy:
+// five VALU instructions.
 __device__ __forceinline__ float splat_weight(bool in, const ConicE2& k, float dx, float dy) {
 #pragma clang fp contract(off)
-  const float x = fmaf(k.a * dx, dx, fmaf(k.b2 * dx, dy, k.c * dy * dy));
+  const float x = fmaf(dx, fmaf(k.b2, dy, k.a * dx), (k.c * dy) * dy);
   return __builtin_amdgcn_exp2f(in ? x : -__builtin_inff());
 }
 
@@ -430,49 +431,6 @@ __device__ inline float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over each 16-lane row; every lane of the row receives the row sum.
-__device__ inline float row_sum16(float v) {
-  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_mov<0x141>(v);  // row_half_mirror
-  v += dpp_mov<0x140>(v);  // row_mirror
-  return v;
-}
-
-template <int CTRL, int ROW_MASK>
-__device__ inline float dpp_rows(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
-}
-
-// Sum over the 64 lanes of a wave; lane 63 receives the total. Row sums by
-// quad/mirror DPP, then row_bcast:15 adds row 0 into row 1 and row 2 into
-// row 3, and row_bcast:31 adds lane 31 into rows 2 and 3.
-__device__ inline float wave_sum_to_last(float v) {
-  v = row_sum16(v);
-  v += dpp_rows<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
-  v += dpp_rows<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
-  return v;
-}
-
-// wave_sum_to_last over N values at once, level by level, so consecutive DPP
-// reads of one value are N instructions apart (no wait states between them).
-template <int N>
-__device__ inline void wave_sums_to_last(float (&v)[N]) {
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0xB1>(v[c]);
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x4E>(v[c]);
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x141>(v[c]);
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_mov<0x140>(v[c]);
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x142, 0xA>(v[c]);
-#pragma unroll
-  for (int c = 0; c < N; ++c) v[c] += dpp_rows<0x143, 0xC>(v[c]);
-}
-
 // v_permlane32_swap: the upper 32 lanes of a trade places with the lower 32
 // lanes of b (one instruction moves both directions).
 __device__ inline void swap_wave_halves(float& a, float& b) {
@@ -491,41 +449,63 @@ __device__ inline void swap_wave_rows(float& a, float& b) {
   b = __uint_as_float(r[1]);
 }
 
-// Which of the eight values wave_sum8_scatter leaves in lane l:
-// 4 b3 + 2 b4 + b5, b_k = bit k of l (every lane holds one total).
+// Which of the eight paired values wave_sum9_scatter leaves in lane l:
+// 4 b3 + 2 b4 + b5, b_k = bit k of l.
 __device__ inline int sum8_slot(int lane) {
   return ((lane >> 3) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 5) & 1);
 }
 
-// Sums of 8 values over the wave, as a reduce-scatter. Value pairs are
+// Sums of 9 values over the wave, as a reduce-scatter. Value pairs are
 // exchanged between the wave halves (v_permlane32_swap: one swap and one add
 // per pair leave the first value's partial sums in the lower half and the
 // second's in the upper), then between 16-lane rows (v_permlane16_swap), then
 // between the two 8-lane halves of each row (a row rotate by 8 with a
-// select); three DPP levels finish the sums inside each 8-lane group: 18
-// VALU instructions for the eight totals, no LDS. Lane l returns the total of
-// value sum8_slot(l).
-__device__ inline float wave_sum8_scatter(const float (&v)[8], int lane) {
-  float s[4];
+// select); three DPP levels finish the sums inside each 8-lane group. The
+// ninth value rides along paired with itself (both halves / rows receive the
+// same partial sum), so every lane ends with its total: ~29 VALU
+// instructions for the nine totals, no LDS. Returns the total of value
+// sum8_slot(l) in `t8` (lanes with l % 8 == 0 hold distinct values) and the
+// total of v[8] in `t9` (every lane).
+__device__ inline void wave_sum9_scatter(const float (&v)[9], int lane, float& t8, float& t9) {
+  float s[5];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     float a = v[2 * k], b = v[2 * k + 1];
     swap_wave_halves(a, b);
     s[k] = a + b;
   }
-  float t[2];
+  {
+    float a = v[8], b = v[8];
+    swap_wave_halves(a, b);
+    s[4] = a + b;
+  }
+  float t[3];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     float a = s[2 * k], b = s[2 * k + 1];
     swap_wave_rows(a, b);
     t[k] = a + b;
   }
+  {
+    float a = s[4], b = s[4];
+    swap_wave_rows(a, b);
+    t[2] = a + b;
+  }
   const bool b3 = lane & 8;
   float u = (b3 ? t[1] : t[0]) + dpp_mov<0x128>(b3 ? t[0] : t[1]);  // row_ror:8
+  float w = t[2] + dpp_mov<0x128>(t[2]);
   u += dpp_mov<0xB1>(u);  // quad_perm [1,0,3,2]
+  w += dpp_mov<0xB1>(w);
   u += dpp_mov<0x4E>(u);  // quad_perm [2,3,0,1]
+  w += dpp_mov<0x4E>(w);
   u += dpp_mov<0x141>(u);  // row_half_mirror
-  return u;
+  w += dpp_mov<0x141>(w);
+  // Both totals are formed here, before the callers' lane-masked stores: an
+  // empty asm that reads them keeps the last DPP add from being sunk into
+  // the masked block (where it would not fuse with its DPP move).
+  asm volatile("" ::"v"(u), "v"(w));
+  t8 = u;
+  t9 = w;
 }
 
 // Reduced per list entry: S_x, S_y, S_xx, S_xy, S_yy (moments of dL/dq over
@@ -727,20 +707,19 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         }
       }
       if (!touched) continue;
-      const float v8[8] = {v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
-      const float tot = wave_sum8_scatter(v8, lane);
-      float vb[1] = {v[8]};
-      wave_sums_to_last(vb);
-      // lanes 0, 8, ..., 56 hold the eight totals, lane 63 the ninth
+      float tot, tot9;
+      wave_sum9_scatter(v, lane, tot, tot9);
+      // lanes 0, 8, ..., 56 hold the eight paired totals, lane 1 writes the
+      // ninth (every lane holds it)
       const bool slot_lane = (lane & 7) == 0;
       if (W == 1 || S > 1) {
         if (slot_lane) acc[sum8_slot(lane)][j] = tot;
-        if (lane == 63) acc[8][j] = vb[0];
+        if (lane == 1) acc[8][j] = tot9;
       } else {
         // one atomic with a per-lane slot: a lane-uniform address would be
         // expanded by the compiler into a loop over the active lanes
-        if (slot_lane || lane == 63)
-          atomicAdd(&acc[slot_lane ? sum8_slot(lane) : 8][j], slot_lane ? tot : vb[0]);
+        if (slot_lane || lane == 1)
+          atomicAdd(&acc[slot_lane ? sum8_slot(lane) : 8][j], slot_lane ? tot : tot9);
       }
     }
     __syncthreads();
