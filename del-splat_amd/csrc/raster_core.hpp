@@ -95,7 +95,7 @@ __device__ __forceinline__ float splat_weight(bool in, const ConicE2& k, float d
 template <int TS, int NT>
 struct GroupMap {
   static constexpr int W = NT / 64;
-  static constexpr int RS = (TS / 16) * (TS / 16) >= W ? 16 : 8;
+  static constexpr int RS = (TS / 16) * (TS / 16) > W ? 16 : 8;
   static constexpr int QS = RS / 8;
   static constexpr int Q = QS * QS;
   static constexpr int RXN = TS / RS;
