@@ -71,14 +71,17 @@ sum_partials_kernel(uint32_t nb, const uint32_t* __restrict__ partial, uint32_t*
   if (threadIdx.x == 0) *d_k = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-// Pairs of the j-th Gaussian in depth order at off[j]: tile ids ix-outer,
-// iy-inner (tile_acceleration.h:29-50), value = Gaussian index. n_eff = K if
-// K <= capacity else 0 (overflow: nothing is written and the later phases
-// see no pairs); workgroup 0 stores it for them. The rectangles come in depth
-// order from the sort (read in sequence, no gather of boxes).
-__global__ void __launch_bounds__(256)
+// Pairs of the j-th Gaussian in depth order from its first slot, the
+// exclusive scan of the counts cnt (in depth order): the chunk prefix of the
+// workgroup's 256 Gaussians plus a block scan. Tile ids ix-outer, iy-inner
+// (tile_acceleration.h:29-50), value = Gaussian index. n_eff = K if K <=
+// capacity else 0 (overflow: nothing is written and the later phases see no
+// pairs); workgroup 0 stores it for them. The rectangles come in depth order
+// from the sort (read in sequence, no gather of boxes).
+__global__ void __launch_bounds__(kScanBlock)
 bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
-                const uint2* __restrict__ rect_sorted, const uint32_t* __restrict__ off,
+                const uint2* __restrict__ rect_sorted, const uint32_t* __restrict__ cnt,
+                const uint32_t* __restrict__ chunk_prefix,
                 const uint32_t* __restrict__ d_k, uint32_t capacity,
                 uint32_t* __restrict__ n_eff_out, uint32_t tw,
                 uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ pnt_of_pair) {
@@ -89,13 +92,15 @@ bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
   if (n_eff == 0) return;  // uniform
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < n;
-  uint32_t g = 0, o = 0;
+  uint32_t g = 0, c = 0;
   TileRect r{0, 0, 0, 0};
   if (valid) {
     g = order[j];
     r = unpack_rect(rect_sorted[j]);  // gathered into depth order by the sort
-    o = off[j];
+    c = cnt[j];
   }
+  uint32_t block_total;
+  const uint32_t o = chunk_prefix[blockIdx.x] + block_exclusive_scan(c, &block_total);
   wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair);
 }
 
@@ -135,7 +140,7 @@ struct PointLayout {
   uint32_t* order;     // [N] Gaussians in depth order
   void* dsort_ws;      // depth sort
   size_t dsort_bytes;
-  uint32_t* off;       // [N + 1] pair offsets in depth order
+  uint32_t* off;       // [N + 1] pair counts in depth order (the fill scans them)
   uint32_t* scan_ws;
   uint32_t* misc;      // [4]: n_eff
   size_t total;
@@ -160,7 +165,7 @@ inline PointLayout point_layout(void* base, uint32_t n) {
   L.dsort_bytes = sort_workspace_bytes<uint32_t>(n);
   L.dsort_ws = take(L.dsort_bytes);
   L.off = reinterpret_cast<uint32_t*>(take((size_t(n) + 1) * 4));
-  L.scan_ws = reinterpret_cast<uint32_t*>(take(scan_workspace_bytes(n)));
+  L.scan_ws = reinterpret_cast<uint32_t*>(take(scan_chunks_workspace_bytes(n)));
   L.misc = reinterpret_cast<uint32_t*>(take(16));
   L.total = o;
   return L;
@@ -203,8 +208,9 @@ int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, u
 // Phase 2: stable depth order of the Gaussians (values generated as iota by
 // the first pass; the last pass also gathers the tile rectangles and their
 // areas, the pair counts, into depth order),
-// then the exclusive scan of the counts = each Gaussian's first pair slot;
-// d_total (may be null) receives the scan's total, K.
+// then the first level of the exclusive scan of the counts (each Gaussian's
+// first pair slot), per 256-Gaussian chunk: the fill finishes it inside its
+// workgroup. d_total (may be null) receives the scan's total, K.
 inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* d_total = nullptr) {
   if (n == 0) {
     if (d_total) set_u32_one<<<1, 1, 0, s>>>(d_total, 0u);
@@ -218,7 +224,7 @@ inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* 
   if (rc) return rc;
   if (in_alt)  // 31 key bits take 4 passes: the result is in place
     return dsplat_set_error(DSPLAT_ERR_ARG, "binning: depth order in the alternate buffer");
-  return exclusive_scan(s, n, nullptr, L.off, L.off, L.scan_ws, d_total);
+  return scan_chunk_prefixes(s, n, L.off, L.scan_ws, d_total);
 }
 
 // Phase 3: pairs in depth order, stable sort by tile id, tile ranges. Writes
@@ -231,8 +237,9 @@ int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, u
   if (n == 0) {
     set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
   } else {
-    bin_fill_kernel<<<div_up(n, 256), 256, 0, s>>>(n, L.order, L.rect_sorted, L.off, d_num_idx,
-                                                   capacity, L.misc, tw, Q.tilekey, idx2pnt);
+    bin_fill_kernel<<<div_up(n, kScanBlock), kScanBlock, 0, s>>>(
+        n, L.order, L.rect_sorted, L.off, L.scan_ws, d_num_idx, capacity, L.misc, tw, Q.tilekey,
+        idx2pnt);
   }
   const uint32_t* tsorted = Q.tilekey;
   if (capacity > 0 && nt > 1) {

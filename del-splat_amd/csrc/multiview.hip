@@ -205,25 +205,30 @@ __global__ void mv_clamp_kernel(uint32_t* __restrict__ misc, uint32_t capacity) 
   misc[4] = k > capacity ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256)
+// First pair slot of item j (depth order): the chunk prefix of its
+// workgroup's 256 items plus a block scan of the counts cnt.
+__global__ void __launch_bounds__(kScanBlock)
 mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views,
                const uint32_t* __restrict__ order, const uint2* __restrict__ rect_sorted,
-               const uint32_t* __restrict__ off, const uint32_t* __restrict__ misc,
+               const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ chunk_prefix,
+               const uint32_t* __restrict__ misc,
                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ item_of_pair) {
   __shared__ ExpandLds lds[4];
   if (misc[1] == 0) return;  // uniform: overflow, nothing is written
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = j < nitems;
-  uint32_t item = 0, o = 0, tbase = 0, tw = 1;
+  uint32_t item = 0, c = 0, tbase = 0, tw = 1;
   TileRect r{0, 0, 0, 0};
   if (valid) {
     item = order[j];
     const ViewDev& vd = views[item / n];
     r = unpack_rect(rect_sorted[j]);  // gathered into depth order by the sort
-    o = off[j];
+    c = cnt[j];
     tbase = vd.tile_off;
     tw = vd.tw;
   }
+  uint32_t block_total;
+  const uint32_t o = chunk_prefix[blockIdx.x] + block_exclusive_scan(c, &block_total);
   wave_expand_pairs(valid, o, r, tbase, tw, item, lds[threadIdx.x >> 6], tile_of_pair,
                     item_of_pair);
 }
@@ -548,7 +553,7 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   P->dsort_bytes = sort_workspace_bytes<uint32_t>(uint32_t(items));
   const size_t o_dsort = take(P->dsort_bytes);
   const size_t o_off = take((items + 1) * 4);
-  const size_t o_scan = take(scan_workspace_bytes(uint32_t(items)));
+  const size_t o_scan = take(scan_chunks_workspace_bytes(uint32_t(items)));
   const size_t o_tkey = take(size_t(pair_capacity) * 4 + 4);
   const size_t o_i2p = take(size_t(pair_capacity) * 4 + 4);
   P->tsort_bytes = sort_workspace_bytes<uint32_t>(pair_capacity);
@@ -795,12 +800,12 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
                                       P->off, P->rect_sorted);
   if (rc) return rc;
   const uint32_t* order = in_alt ? valt : P->order;
-  rc = exclusive_scan(s, items, nullptr, P->off, P->off, P->scan_ws, P->misc);
+  rc = scan_chunk_prefixes(s, items, P->off, P->scan_ws, P->misc);
   if (rc) return rc;
   mv_clamp_kernel<<<1, 1, 0, s>>>(P->misc, P->capacity);
-  mv_fill_kernel<<<div_up(items, 256), 256, 0, s>>>(items, n, P->d_views, order,
-                                                     P->rect_sorted, P->off, P->misc,
-                                                     P->tilekey, P->idx2pnt);
+  mv_fill_kernel<<<div_up(items, kScanBlock), kScanBlock, 0, s>>>(
+      items, n, P->d_views, order, P->rect_sorted, P->off, P->scan_ws, P->misc, P->tilekey,
+      P->idx2pnt);
   const uint32_t* tsorted = P->tilekey;
   if (P->capacity > 0 && P->total_tiles > 1) {
     rc = radix_sort_pairs<uint32_t>(s, P->capacity, P->misc + 1, P->tilekey, P->idx2pnt,

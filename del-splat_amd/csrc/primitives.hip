@@ -16,32 +16,11 @@
 namespace dsplat {
 
 // ------------------------------------------------------------------ scan ----
-constexpr int kScanBlock = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 // DPP row shifts and broadcasts: no LDS round trips.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v) { return wave_incl_sum_u32(v); }
-
-// Exclusive scan of one value per thread over a 256-thread block; returns
-// the block total in *total.
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-  __shared__ uint32_t wsum[kScanBlock / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t inc = wave_inclusive_scan(v);
-  if (lane == 63) wsum[w] = inc;
-  __syncthreads();
-  uint32_t pre = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < kScanBlock / 64; ++k) {
-    const uint32_t s = wsum[k];
-    if (k < w) pre += s;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return pre + inc - v;
-}
 
 __device__ inline uint32_t load_n(uint32_t n_cap, const uint32_t* d_n) {
   if (!d_n) return n_cap;
@@ -49,13 +28,15 @@ __device__ inline uint32_t load_n(uint32_t n_cap, const uint32_t* d_n) {
   return v < n_cap ? v : n_cap;
 }
 
+template <int ITEMS>
 __global__ void __launch_bounds__(kScanBlock)
 scan_reduce_kernel(uint32_t n_cap, const uint32_t* d_n, const uint32_t* __restrict__ in,
                    uint32_t* __restrict__ partial) {
   const uint32_t n = load_n(n_cap, d_n);
-  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  const uint64_t base = uint64_t(blockIdx.x) * (kScanBlock * ITEMS);
   uint32_t s = 0;
-  for (int k = 0; k < kScanItems; ++k) {
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
     const uint64_t i = base + uint64_t(k) * kScanBlock + threadIdx.x;
     if (i < n) s += in[i];
   }
@@ -65,16 +46,28 @@ scan_reduce_kernel(uint32_t n_cap, const uint32_t* d_n, const uint32_t* __restri
 }
 
 // Single workgroup: exclusive scan of nb partial sums, in place; also writes
-// the grand total to partial[nb].
+// the grand total to partial[nb]. 16 consecutive values per thread: one block
+// scan per 4096 values.
 __global__ void __launch_bounds__(kScanBlock)
 scan_partials_kernel(uint32_t nb, uint32_t* __restrict__ partial, uint32_t* d_total) {
+  constexpr uint32_t kPer = 16, kSpan = kScanBlock * kPer;
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kScanBlock) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < nb ? partial[i] : 0u;
+  for (uint32_t base = 0; base < nb; base += kSpan) {
+    const uint32_t i0 = base + threadIdx.x * kPer;
+    uint32_t v[kPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      v[k] = i0 + k < nb ? partial[i0 + k] : 0u;
+      sum += v[k];
+    }
     uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, &tot);
-    if (i < nb) partial[i] = carry + ex;
+    uint32_t run = carry + block_exclusive_scan(sum, &tot);
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+      if (i0 + k < nb) partial[i0 + k] = run;
+      run += v[k];
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) {
@@ -121,9 +114,27 @@ int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uin
     return DSPLAT_OK;
   }
   const uint32_t nb = div_up(n_cap, kScanTile);
-  scan_reduce_kernel<<<nb, kScanBlock, 0, s>>>(n_cap, d_n, in, ws);
+  scan_reduce_kernel<kScanItems><<<nb, kScanBlock, 0, s>>>(n_cap, d_n, in, ws);
   scan_partials_kernel<<<1, kScanBlock, 0, s>>>(nb, ws, d_total);
   scan_down_kernel<<<nb, kScanBlock, 0, s>>>(n_cap, d_n, in, out, ws);
+  DSPLAT_HIP(hipGetLastError());
+  return DSPLAT_OK;
+}
+
+size_t scan_chunks_workspace_bytes(uint32_t n) {
+  return (size_t(div_up(n, kScanBlock)) + 2) * sizeof(uint32_t) + 256;
+}
+
+int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t* ws,
+                        uint32_t* d_total) {
+  if (n == 0) {
+    if (d_total) set_u32_kernel<<<1, 1, 0, s>>>(d_total, 0u);
+    DSPLAT_HIP(hipGetLastError());
+    return DSPLAT_OK;
+  }
+  const uint32_t nb = div_up(n, kScanBlock);
+  scan_reduce_kernel<1><<<nb, kScanBlock, 0, s>>>(n, nullptr, in, ws);
+  scan_partials_kernel<<<1, kScanBlock, 0, s>>>(nb, ws, d_total);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }

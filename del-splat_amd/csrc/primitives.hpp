@@ -4,7 +4,31 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "dsplat_common.hpp"
+
 namespace dsplat {
+
+constexpr int kScanBlock = 256;
+
+// Exclusive scan of one value per thread over a kScanBlock-thread block;
+// returns the block total in *total. Every thread of the block must call it.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kScanBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_sum_u32(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 64; ++k) {
+    const uint32_t s = wsum[k];
+    if (k < w) pre += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + inc - v;
+}
 
 // Workspace bytes of exclusive_scan for n elements.
 size_t scan_workspace_bytes(uint32_t n);
@@ -14,6 +38,15 @@ size_t scan_workspace_bytes(uint32_t n);
 // written there. ws: scan_workspace_bytes(n_cap) bytes.
 int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uint32_t* in,
                    uint32_t* out, uint32_t* ws, uint32_t* d_total);
+
+// The first level of an exclusive scan, left for a consumer to finish:
+// ws[b] = in[0] + ... + in[256 b - 1] for the 256-element chunks b of
+// in[0..n), ws[nb] = the total (also written to d_total if not null). The
+// consumer finishes chunk b with one block_exclusive_scan over its 256
+// elements. ws: scan_chunks_workspace_bytes(n) bytes.
+size_t scan_chunks_workspace_bytes(uint32_t n);
+int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t* ws,
+                        uint32_t* d_total);
 
 template <typename KeyT>
 size_t sort_workspace_bytes(uint32_t n);
