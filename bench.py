@@ -312,22 +312,37 @@ def train(args, torch, np, SG, scene):
     m2 = torch.zeros_like(params)
     dstep = torch.zeros(1, device=dev, dtype=torch.int32)
     max_pix = nview * 1920 * 1080
-    plan = None
-    ts, th = 32, 256
 
     def views_for(step):
         return scene.sphere_cameras(nview, 1000 + step)
 
-    # gradient agreement on step 0: batched adjoint vs the sum over views of
-    # the single-view adjoint (both before any update)
     views = views_for(0)
-    plan = MultiViewPlan(n, views, ts, th, max_pixels=max_pix,
-                         pair_capacity=int(n * nview * 14))
     targets = torch.cat([torch.from_numpy(scene.target_image(sh, 5000 + v)).to(dev).reshape(-1)
                          for v, (_, sh) in enumerate(views)])
     xyz, ls, q = params[:3 * n].view(n, 3), params[3 * n:6 * n].view(n, 3), params[6 * n:10 * n].view(n, 4)
     op = torch.sigmoid(params[10 * n:11 * n])
     rgb = params[11 * n:].view(n, 3)
+    # workgroup shape for this workload (the plan's forward and adjoint share
+    # it): the faster fused forward + adjoint over the first step's views
+    tune = {}
+    for ts_c, th_c in ((32, 256), (32, 512)):
+        p = MultiViewPlan(n, views, ts_c, th_c, max_pixels=max_pix,
+                          pair_capacity=int(n * nview * 14))
+        for rep in range(3):  # the first run warms up
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            o, _ = p.forward(xyz, ls, q, op, rgb)
+            p.backward(xyz, ls, q, op, out_rgb=o, target=targets)
+            torch.cuda.synchronize()
+            tune[f"{ts_c}/{th_c}"] = round((time.perf_counter() - t1) * 1000.0, 3)
+        p.close()
+        del o
+    ts, th = (int(v) for v in min(tune, key=tune.get).split("/"))
+
+    # gradient agreement on step 0: batched adjoint vs the sum over views of
+    # the single-view adjoint (both before any update)
+    plan = MultiViewPlan(n, views, ts, th, max_pixels=max_pix,
+                         pair_capacity=int(n * nview * 14))
     out, _ = plan.forward(xyz, ls, q, op, rgb)
     k0, over = plan.counters()
     if over:
@@ -369,7 +384,7 @@ def train(args, torch, np, SG, scene):
     plan.close()
     torch.cuda.empty_cache()
     return {"num_gauss": n, "views_per_step": nview, "steps": steps, "ms_per_step": round(ms, 3),
-            "tile": ts, "threads": th, "pairs_K_step0": k0, "overflow": over,
+            "tile": ts, "threads": th, "autotune_ms": tune, "pairs_K_step0": k0, "overflow": over,
             "grad_max_rel_diff_vs_per_view": {k2: round(v, 7) for k2, v in agree.items()},
             "loss_first_last": losses,
             "note": "step time includes the per-step host work (view table upload, target "
