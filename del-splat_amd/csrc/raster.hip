@@ -60,13 +60,13 @@ int launch_fwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
     DSPLAT_FWD(8, 64);
   } else if (ts == 16) {
     if (threads == 64) DSPLAT_FWD(16, 64);
-    else if (threads == 128) DSPLAT_FWD(16, 128);
-    else DSPLAT_FWD(16, 256);
+    else if (threads == 256) DSPLAT_FWD(16, 256);
+    else DSPLAT_FWD(16, 128);
   } else if (ts == 32) {
     if (threads == 1024) DSPLAT_FWD(32, 1024);
-    else if (threads == 512) DSPLAT_FWD(32, 512);
+    else if (threads == 256) DSPLAT_FWD(32, 256);
     else if (threads == 128) DSPLAT_FWD(32, 128);
-    else DSPLAT_FWD(32, 256);
+    else DSPLAT_FWD(32, 512);
   } else {
     return dsplat_set_error(DSPLAT_ERR_ARG, "tile_size must be 8, 16 or 32");
   }

@@ -156,8 +156,9 @@ int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xy
  * pixel -- the terminating one if the walk terminated --, 0 if none),
  * pix2tlast [H,W] (transmittance in front of that splat, 1 if none).
  * pix2alpha / pix2nwalk / pix2tlast may be null. `threads` selects the
- * workgroup size per tile (0 = default; ts 16: 64/128/256, ts 32:
- * 128/256/512/1024; ts 8: always 64). */
+ * workgroup size per tile (0 = default; ts 16: 64/128/256, default 128;
+ * ts 32: 128/256/512/1024, default 512; ts 8: always 64). Every size
+ * renders bit-identical pixels. */
 int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
                                uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                const uint32_t* tile2idx, const uint32_t* idx2pnt,
