@@ -462,35 +462,36 @@ __device__ inline int sum8_slot(int lane) {
 // between the two 8-lane halves of each row (a row rotate by 8 with a
 // select); three DPP levels finish the sums inside each 8-lane group. The
 // ninth value rides along paired with itself (both halves / rows receive the
-// same partial sum), so every lane ends with its total: ~29 VALU
+// same partial sum), so every lane ends with its total: 28 VALU
 // instructions for the nine totals, no LDS. Returns the total of value
 // sum8_slot(l) in `t8` (lanes with l % 8 == 0 hold distinct values) and the
 // total of v[8] in `t9` (every lane).
 __device__ inline void wave_sum9_scatter(const float (&v)[9], int lane, float& t8, float& t9) {
-  float s[5];
+  // all swaps of a level before its adds: the in-place swap operands then
+  // need no extra register copies
+  float a[5], b[5];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    float a = v[2 * k], b = v[2 * k + 1];
-    swap_wave_halves(a, b);
-    s[k] = a + b;
+    a[k] = v[2 * k];
+    b[k] = v[2 * k + 1];
   }
-  {
-    float a = v[8], b = v[8];
-    swap_wave_halves(a, b);
-    s[4] = a + b;
-  }
+  a[4] = b[4] = v[8];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) swap_wave_halves(a[k], b[k]);
+  float s[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) s[k] = a[k] + b[k];
+  float c[3], d[3];
+  c[0] = s[0];
+  d[0] = s[1];
+  c[1] = s[2];
+  d[1] = s[3];
+  c[2] = d[2] = s[4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) swap_wave_rows(c[k], d[k]);
   float t[3];
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    float a = s[2 * k], b = s[2 * k + 1];
-    swap_wave_rows(a, b);
-    t[k] = a + b;
-  }
-  {
-    float a = s[4], b = s[4];
-    swap_wave_rows(a, b);
-    t[2] = a + b;
-  }
+  for (int k = 0; k < 3; ++k) t[k] = c[k] + d[k];
   const bool b3 = lane & 8;
   float u = (b3 ? t[1] : t[0]) + dpp_mov<0x128>(b3 ? t[0] : t[1]);  // row_ror:8
   float w = t[2] + dpp_mov<0x128>(t[2]);
