@@ -375,7 +375,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
                 tl[k] = in ? tr[k] : tl[k];
                 walked[k] = in ? b + j + 1 : walked[k];
-                tr[k] = tr[k] - eo;  // T (1 - e) as T - T e: one instruction, >= 0
+                tr[k] = tr[k] * (1.f - e);
                 asum[k] = asum[k] + eo;
               }
             }
