@@ -449,6 +449,10 @@ struct dsplat_mv_plan {
   const float* fwd_sh = nullptr;
   void* sh_mem = nullptr;
   bool cams_ok = false;
+  // a forward over the current views and colour mode has been enqueued (or
+  // replayed from a graph): the adjoint reads its lists, walks and
+  // transmittances. Cleared by set_views, set_sh and a failed forward.
+  bool fwd_valid = false;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap_stream = nullptr;
 };
@@ -628,6 +632,7 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
                                    const dsplat_view_desc* views) {
   if (!P || !views) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan/views");
   drop_graph(P);
+  P->fwd_valid = false;
   if (int rc = fill_views(P, num_view, views)) return rc;
   DSPLAT_HIP(hipMemcpyAsync(P->d_views, P->host_views.data(), sizeof(ViewDev) * num_view,
                             hipMemcpyHostToDevice, as_stream(stream)));
@@ -639,6 +644,7 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
 extern "C" int dsplat_mv_set_sh(dsplat_mv_plan* P, uint32_t degree, uint32_t num_coef) {
   if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
   drop_graph(P);
+  P->fwd_valid = false;
   if (num_coef == 0) {
     P->sh_k = P->sh_deg = 0;
     return DSPLAT_OK;
@@ -757,9 +763,19 @@ static size_t param_count(const dsplat_mv_plan* P) {
 
 static bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// Under SH colour the flat gradient block has padding between the opacity
+// block (ending at 11N) and the colour block (colour_offset): the adjoint
+// never writes it, so it is zeroed here (Adam then leaves the padding's
+// parameters and moments unchanged, whatever the caller allocated).
+static void zero_grad_padding(const dsplat_mv_plan* P, hipStream_t s, float* grads) {
+  const size_t o = 11 * size_t(P->n), coff = colour_offset(P);
+  if (coff > o) zero_f32_kernel<<<1, 256, 0, s>>>(coff - o, grads + o);
+}
+
 static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
                       const float* quat, const float* opacity, const float* rgb,
                       uint32_t flags, float* out_rgb, float* out_alpha) {
+  P->fwd_valid = false;
   if (!out_alpha) out_alpha = P->alpha;
   const uint32_t n = P->n, nv = P->nview;
   const uint32_t items = n * nv;
@@ -831,7 +847,9 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   else launch_mv_fwd<32, 256>(P, s, out_rgb, out_alpha);
   DSPLAT_HIP(hipGetLastError());
   // longest walks first for the adjoint and the next forward
-  return order_by_cost(s, P->total_tiles, P->tile_cost, P->tile_order);
+  rc = order_by_cost(s, P->total_tiles, P->tile_cost, P->tile_order);
+  P->fwd_valid = rc == DSPLAT_OK;
+  return rc;
 }
 
 static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const float* scale,
@@ -841,6 +859,9 @@ static int mv_backward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const
                        uint32_t flags) {
   const uint32_t n = P->n, nv = P->nview;
   const size_t items = size_t(n) * nv;
+  if (!P->fwd_valid)
+    return dsplat_set_error(DSPLAT_ERR_ARG,
+                            "mv_backward: no forward over the current views and colour mode");
   if (P->sh_k && (P->fwd_sh_k != P->sh_k || P->fwd_sh_deg != P->sh_deg || !P->fwd_sh))
     return dsplat_set_error(DSPLAT_ERR_ARG,
                             "mv_backward: the last forward did not use the current SH colour");
@@ -952,6 +973,7 @@ static int mv_train_step(dsplat_mv_plan* P, hipStream_t s, float* params, float*
   rc = mv_backward(P, s, xyz, ls, q, op, out_rgb, nullptr, targets, nullptr, grads,
                    grads + 3 * n, grads + 6 * n, grads + 10 * n, grads + coff, flags);
   if (rc) return rc;
+  zero_grad_padding(P, s, grads);
   adam_tick_kernel<<<1, 1, 0, s>>>(d_step);
   adam_kernel<<<grid_cap(total), 256, 0, s>>>(total, params, grads, m1, m2, lr, b1, b2, eps,
                                               d_step);
@@ -991,6 +1013,9 @@ extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const f
     P->exec = nullptr;
   }
   hipGraph_t graph = nullptr;
+  // capturing runs nothing: the forward-valid state is that of the last
+  // executed work (a replay sets it)
+  const bool fwd_valid = P->fwd_valid;
   DSPLAT_HIP(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
   int rc;
   if (kind == 0) {
@@ -1008,6 +1033,7 @@ extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const f
       rc = mv_backward(P, P->cap_stream, xyz, scale, quat, opacity, out_rgb, nullptr, targets,
                        nullptr, grads, grads + 3 * n, grads + 6 * n, grads + 10 * n,
                        grads + colour_offset(P), flags);
+    if (!rc) zero_grad_padding(P, P->cap_stream, grads);
   } else {
     rc = (params && grads && m1 && m2 && d_step && targets)
              ? mv_train_step(P, P->cap_stream, params, grads, m1, m2, d_step, targets, out_rgb,
@@ -1015,6 +1041,7 @@ extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const f
              : dsplat_set_error(DSPLAT_ERR_ARG, "graph: null training argument");
   }
   hipError_t e = hipStreamEndCapture(P->cap_stream, &graph);
+  P->fwd_valid = fwd_valid;
   if (rc) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
@@ -1029,5 +1056,8 @@ extern "C" int dsplat_mv_graph_capture(dsplat_mv_plan* P, uint32_t kind, const f
 extern "C" int dsplat_mv_graph_launch(dsplat_mv_plan* P, void* stream) {
   if (!P || !P->exec) return dsplat_set_error(DSPLAT_ERR_ARG, "graph: nothing captured");
   DSPLAT_HIP(hipGraphLaunch(P->exec, as_stream(stream)));
+  // every captured kind starts with a forward over the current views (a
+  // change of views or colour mode drops the graph)
+  P->fwd_valid = true;
   return DSPLAT_OK;
 }

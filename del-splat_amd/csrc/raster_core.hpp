@@ -61,28 +61,32 @@ struct ArrayLoader {
 };
 
 // The splat weight exp(-q/2), q = d^T C d for d = (dx, dy) and C = [[a, b],
-// [b, c]], as 2^(d^T C' d) with C' = -log2(e)/2 C formed once per list entry
-// (wave-uniform). One fixed evaluation order (contraction off; fused only
-// where fmaf is written), shared by the forward and the adjoint, so every
-// workgroup shape produces bit-identical pixels and the adjoint recomputes
-// exactly the forward's weights.
-struct ConicE2 {
-  float a, b2, c;  // scaled a, 2b, c
+// [b, c]]. q is rounded exactly as the reference's CPU evaluation writes it,
+// a dx dx + b (dx dy + dy dx) + c dy dy left to right with every product
+// and sum rounded (splat_gauss.cu:97 / pnt2pixxyndcz.rs:85, contraction
+// off): b (dx dy + dy dx) is formed as (2b) (dx dy), which rounds the same
+// (dx dy + dy dx = 2 (dx dy) exactly, and scaling by 2 commutes with
+// rounding). So q carries the same bits as the oracle's even where its terms
+// cancel (flat, elongated splats: q is then a small difference of large
+// terms, and any other order moves it), and only the exponential differs:
+// 2^(q * -log2(e)/2), a few ulps from expf(-q/2). Shared by the forward and
+// the adjoint, so every workgroup shape produces bit-identical pixels and the
+// adjoint recomputes exactly the forward's weights.
+struct ConicQ {
+  float a, b2, c;  // a, 2b, c
 };
 
-__device__ __forceinline__ ConicE2 conic_e2(float a, float b, float c) {
-#pragma clang fp contract(off)
-  constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
-  return {a * kS, (b + b) * kS, c * kS};
+__device__ __forceinline__ ConicQ conic_q(float a, float b, float c) {
+  return {a, b + b, c};
 }
 
 // exp(-q/2) where in, else 0 (2^-inf = 0: a select on the argument keeps
-// the exp unpredicated). The exponent as dx (b2' dy + a' dx) + (c' dy) dy:
-// five VALU instructions.
-__device__ __forceinline__ float splat_weight(bool in, const ConicE2& k, float dx, float dy) {
+// the exp unpredicated).
+__device__ __forceinline__ float splat_weight(bool in, const ConicQ& k, float dx, float dy) {
 #pragma clang fp contract(off)
-  const float x = fmaf(dx, fmaf(k.b2, dy, k.a * dx), (k.c * dy) * dy);
-  return __builtin_amdgcn_exp2f(in ? x : -__builtin_inff());
+  constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
+  const float q = ((k.a * dx) * dx + k.b2 * (dx * dy)) + (k.c * dy) * dy;
+  return __builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff());
 }
 
 // Pixel -> thread mapping of a tile (forward and adjoint). The TS x TS tile
@@ -317,8 +321,8 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     if (threadIdx.x < uint32_t(B) && it < num) {
       float4 q0, q1, q2, q3;
       ld.load(idx2pnt[end - 1 - it], q0, q1, q2, q3);
-      // the conic scaled for exp2 once per entry (every wave uses it)
-      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      // the conic with b doubled once per entry (every wave uses it)
+      const ConicQ ke = conic_q(q0.z, q0.w, q1.x);
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
       s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
@@ -354,7 +358,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           const float4 q3 = s3[j];
           const float4 q0 = s0[j];
           const float4 q1 = s1[j];
-          const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
+          const ConicQ ke{q0.z, q0.w, q1.x};  // b doubled at staging
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             bool inm[Q];
@@ -624,9 +628,9 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const uint32_t p = idx2pnt[end - 1 - it];
       const float4* r = splat2 + 4 * size_t(p);
       const float4 q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
-      // the conic scaled for exp2 where the walk reads it, the raw conic
-      // in the record's free slots for the flush
-      const ConicE2 ke = conic_e2(q0.z, q0.w, q1.x);
+      // the conic with b doubled where the walk reads it, the raw conic in
+      // the record's free slots for the flush
+      const ConicQ ke = conic_q(q0.z, q0.w, q1.x);
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
       s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
@@ -654,7 +658,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       if (hits == 0) continue;
       const float4 q2 = s2[j], q3 = s3[j];
       const float4 q0 = s0[j], q1 = s1[j];
-      const ConicE2 ke{q0.z, q0.w, q1.x};  // scaled at staging
+      const ConicQ ke{q0.z, q0.w, q1.x};  // b doubled at staging
       // -0 so that the first group's terms enter without an add (x + -0 = x)
       float v[kGrad];
 #pragma unroll

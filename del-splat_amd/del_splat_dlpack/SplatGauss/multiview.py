@@ -55,6 +55,9 @@ class MultiViewPlan:
         self._last_alpha = None
         self._graph_alpha = None
         self.sh_degree, self.sh_coef = 0, 0  # plain rgb until set_sh
+        # a forward over the current views and colour mode has run (the C
+        # ABI keeps the same state and refuses a backward without it)
+        self._fwd_valid = False
         self.set_views(views)
 
     # ------------------------------------------------------------- views --
@@ -74,6 +77,7 @@ class MultiViewPlan:
         self.pix_off = [int(x) for x in po]
         self.tile_off = [int(x) for x in to]
         self.graph_kind = None
+        self._fwd_valid = False
 
     def set_sh(self, degree, num_coef=16):
         """Colour from spherical-harmonic coefficients [N, num_coef, 3] seen
@@ -83,6 +87,7 @@ class MultiViewPlan:
         call("dsplat_mv_set_sh", self._plan, int(degree), int(num_coef))
         self.sh_degree, self.sh_coef = (int(degree), int(num_coef)) if num_coef else (0, 0)
         self.graph_kind = None
+        self._fwd_valid = False
 
     def param_layout(self):
         """Offsets of the flat training block: dict of name -> (start, stop)
@@ -126,10 +131,14 @@ class MultiViewPlan:
         call("dsplat_mv_forward", self._plan, stream_ptr(self.device), ptr(xyz), ptr(scale),
              ptr(quat), ptr(opacity), ptr(rgb), flags, ptr(out_rgb), ptr(out_alpha))
         self._last_alpha = out_alpha
+        self._fwd_valid = True
         return out_rgb, out_alpha
 
     def backward(self, xyz, scale, quat, opacity, out_rgb=None, d_img=None, target=None,
                  d_alpha=None, flags=_native.DSPLAT_SCALE_IS_LOG):
+        if not self._fwd_valid:
+            raise ValueError("MultiViewPlan.backward: no forward over the current views and "
+                             "colour mode (call forward, train_step or replay first)")
         n = self.num_pnt
         dev = self.device
         colour_shape = (n, self.sh_coef, 3) if self.sh_coef else (n, 3)
@@ -147,6 +156,7 @@ class MultiViewPlan:
              ptr(m1), ptr(m2), ptr(d_step), ptr(targets), ptr(out_rgb), lr, betas[0], betas[1],
              eps)
         self._last_alpha = None
+        self._fwd_valid = True
 
     # ------------------------------------------------------------ graphs --
     def capture_forward(self, xyz, scale, quat, opacity, rgb, out_rgb, out_alpha,
@@ -178,6 +188,7 @@ class MultiViewPlan:
     def replay(self):
         call("dsplat_mv_graph_launch", self._plan, stream_ptr(self.device))
         self._last_alpha = self._graph_alpha
+        self._fwd_valid = True
 
     # ------------------------------------------------------------ state --
     def _outputs(self):

@@ -405,7 +405,11 @@ int dsplat_mv_forward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
  * [pix,3] or, when d_img is null, from the loss sum (out_rgb - target)^2 with
  * target [pix,3]. d_alpha [pix] may be null. Overwrites d_pnt2xyz [N,3],
  * d_pnt2scale [N,3], d_pnt2quat [N,4], d_pnt2opacity [N], d_pnt2rgb [N,3]
- * (under SH colour: dL/dsh [N][K][3]). */
+ * (under SH colour: dL/dsh [N][K][3]). Returns DSPLAT_ERR_ARG (before any
+ * launch) unless a forward over the current views and colour mode has been
+ * enqueued since the plan was created or last given views / SH mode: a direct
+ * dsplat_mv_forward, a training step or a graph replay (capturing alone runs
+ * nothing). */
 int dsplat_mv_backward(dsplat_mv_plan* plan, void* stream, const float* pnt2xyz,
                        const float* pnt2scale, const float* pnt2quat, const float* pnt2opacity,
                        const float* out_rgb, const float* d_img, const float* target,
@@ -424,7 +428,9 @@ int dsplat_adam_step(void* stream, uint64_t n, float* param, const float* grad, 
  * grads [14N], Adam on all 14N values (moments m1, m2 [14N]). Under SH colour
  * (dsplat_mv_set_sh) the block is [xyz | log-scale | quat | opacity logit |
  * padding to a multiple of 4 floats | sh 3KN], i.e. the coefficients start at
- * 11N rounded up to a multiple of 4, and Adam covers the whole block. */
+ * 11N rounded up to a multiple of 4, and Adam covers the whole block; the
+ * step zeroes the padding's gradients, so the padding parameters stay as
+ * they are. */
 int dsplat_mv_train_step(dsplat_mv_plan* plan, void* stream, float* params, float* grads,
                          float* m1, float* m2, uint32_t* d_step, const float* targets,
                          float* out_rgb, float lr, float beta1, float beta2, float eps);
@@ -432,8 +438,8 @@ int dsplat_mv_train_step(dsplat_mv_plan* plan, void* stream, float* params, floa
 /* Captures one forward (kind 0: the dsplat_mv_forward arguments), one
  * training step (kind 1: the dsplat_mv_train_step arguments) or one forward
  * + L2 adjoint (kind 2: forward arguments, targets, and grads laid out as the
- * training parameter block, 14N values or the SH layout) into the plan's
- * hipGraph;
+ * training parameter block, 14N values or the SH layout, padding gradients
+ * zeroed) into the plan's hipGraph;
  * dsplat_mv_graph_launch replays it on any stream. The pointers are baked
  * into the graph. */
 int dsplat_mv_graph_capture(dsplat_mv_plan* plan, uint32_t kind, const float* pnt2xyz,

@@ -305,51 +305,115 @@ uint64_t tile2pnt(uint32_t n, const float* boxes, const float* pixxydepth,
 // and tlast (transmittance in front of that splat, 1 if none).
 // If frozen_nwalk != null the walk length per pixel is taken from it and the
 // termination test is skipped (the function the adjoint differentiates).
+// exp_ulp != 0 moves every exp result that many ulps up (> 0) or down (< 0):
+// a planted perturbation that measures how many pixels' termination at 0.999
+// is decided by the last bit of the exponential (tests only).
+// One pixel (ix, iy) of the tiled raster over its tile's list
+// idx2pnt[i0 .. i0 + num), walked from the end. lim caps the walk (the
+// frozen length when frozen is set, which also skips the termination test).
+template <typename T>
+struct PixelOut {
+  T c[3];
+  T alpha_sum, tlast;
+  uint32_t walked;
+};
+
+template <typename T>
+inline PixelOut<T> raster_pixel(uint32_t ix, uint32_t iy, const T* pixxydepth, const T* conic,
+                                const float* aabb, const T* opacity, const T* rgb,
+                                const uint32_t* idx2pnt, uint32_t i0, uint32_t num,
+                                uint32_t lim, bool frozen, int exp_ulp) {
+  const float tx = float(ix) + 0.5f, ty = float(iy) + 0.5f;
+  PixelOut<T> o{{0, 0, 0}, 0, 1, 0};
+  T alpha_occu = 1;
+  for (uint32_t it = 0; it < lim; ++it) {
+    const uint32_t p = idx2pnt[i0 + num - 1 - it];
+    const float* bb = aabb + 4 * p;
+    // inclusive box test (aabb2::is_include_point)
+    if (!(tx >= bb[0] && tx <= bb[2] && ty >= bb[1] && ty <= bb[3])) continue;
+    const T dx = T(tx) - pixxydepth[3 * p + 0];
+    const T dy = T(ty) - pixxydepth[3 * p + 1];
+    const T* cn = conic + 3 * p;
+    const T q = cn[0] * dx * dx + cn[1] * (dx * dy + dy * dx) + cn[2] * dy * dy;
+    T g = std::exp(T(-0.5) * q);
+    for (int u = 0; u < (exp_ulp < 0 ? -exp_ulp : exp_ulp); ++u)
+      g = std::nextafter(g, exp_ulp > 0 ? T(INFINITY) : T(-INFINITY));
+    const T e = g * opacity[p];
+    const T eo = alpha_occu * e;
+    o.c[0] += rgb[3 * p + 0] * eo;
+    o.c[1] += rgb[3 * p + 1] * eo;
+    o.c[2] += rgb[3 * p + 2] * eo;
+    o.tlast = alpha_occu;
+    o.walked = it + 1;
+    alpha_occu *= T(1) - e;
+    o.alpha_sum += eo;
+    // the reference compares the float sum against the double 0.999
+    if (!frozen && double(o.alpha_sum) > 0.999) break;
+  }
+  if (frozen) o.walked = lim;
+  return o;
+}
+
 template <typename T>
 void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const T* conic,
                      const float* aabb, const T* opacity, const T* rgb, uint32_t ts,
                      const uint32_t* tile2idx, const uint32_t* idx2pnt,
                      T* pix2rgb, T* pix2alpha, uint32_t* pix2nwalk, T* pix2tlast,
-                     const uint32_t* frozen_nwalk) {
+                     const uint32_t* frozen_nwalk, int exp_ulp = 0) {
   const uint32_t tw = (img_w + ts - 1) / ts;
   for (uint32_t iy = 0; iy < img_h; ++iy) {
     for (uint32_t ix = 0; ix < img_w; ++ix) {
       const uint32_t ipix = iy * img_w + ix;
       const uint32_t tile = (iy / ts) * tw + ix / ts;
-      const float tx = float(ix) + 0.5f, ty = float(iy) + 0.5f;
       const uint32_t i0 = tile2idx[tile], num = tile2idx[tile + 1] - i0;
-      T c[3] = {0, 0, 0};
-      T alpha_sum = 0, alpha_occu = 1, tlast = 1;
-      uint32_t walked = 0;
       const uint32_t lim = frozen_nwalk ? frozen_nwalk[ipix] : num;
-      for (uint32_t it = 0; it < lim; ++it) {
-        const uint32_t p = idx2pnt[i0 + num - 1 - it];
-        const float* bb = aabb + 4 * p;
-        // inclusive box test (aabb2::is_include_point)
-        if (!(tx >= bb[0] && tx <= bb[2] && ty >= bb[1] && ty <= bb[3])) continue;
-        const T dx = T(tx) - pixxydepth[3 * p + 0];
-        const T dy = T(ty) - pixxydepth[3 * p + 1];
-        const T* cn = conic + 3 * p;
-        const T q = cn[0] * dx * dx + cn[1] * (dx * dy + dy * dx) + cn[2] * dy * dy;
-        const T e = std::exp(T(-0.5) * q) * opacity[p];
-        const T eo = alpha_occu * e;
-        c[0] += rgb[3 * p + 0] * eo;
-        c[1] += rgb[3 * p + 1] * eo;
-        c[2] += rgb[3 * p + 2] * eo;
-        tlast = alpha_occu;
-        walked = it + 1;
-        alpha_occu *= T(1) - e;
-        alpha_sum += eo;
-        // the reference compares the float sum against the double 0.999
-        if (!frozen_nwalk && double(alpha_sum) > 0.999) break;
-      }
-      if (frozen_nwalk) walked = lim;
-      for (int k = 0; k < 3; ++k) pix2rgb[3 * ipix + k] = c[k];
-      if (pix2alpha) pix2alpha[ipix] = alpha_sum;
-      if (pix2nwalk) pix2nwalk[ipix] = walked;
-      if (pix2tlast) pix2tlast[ipix] = tlast;
+      const PixelOut<T> o = raster_pixel(ix, iy, pixxydepth, conic, aabb, opacity, rgb, idx2pnt,
+                                         i0, num, lim, frozen_nwalk != nullptr, exp_ulp);
+      for (int k = 0; k < 3; ++k) pix2rgb[3 * ipix + k] = o.c[k];
+      if (pix2alpha) pix2alpha[ipix] = o.alpha_sum;
+      if (pix2nwalk) pix2nwalk[ipix] = o.walked;
+      if (pix2tlast) pix2tlast[ipix] = o.tlast;
     }
   }
+}
+
+// sum (rgb - target)^2 over the pixels of the tiles [tx0, tx1] x [ty0, ty1]
+// (clipped to the image), every walk frozen at frozen_nwalk: the part of the
+// L2 loss that a change of one Gaussian inside those tiles can move, for
+// central differences of one Gaussian at a time on a large frame.
+template <typename T>
+T gauss_l2_loss_tiles(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const T* conic,
+                      const float* aabb, const T* opacity, const T* rgb, uint32_t ts,
+                      const uint32_t* tile2idx, const uint32_t* idx2pnt,
+                      const uint32_t* frozen_nwalk, const float* target, uint32_t tx0,
+                      uint32_t tx1, uint32_t ty0, uint32_t ty1) {
+  const uint32_t tw = (img_w + ts - 1) / ts;
+  const uint32_t y0 = ty0 * ts, y1 = std::min((ty1 + 1) * ts, img_h);
+  const uint32_t x0 = tx0 * ts, x1 = std::min((tx1 + 1) * ts, img_w);
+  if (y1 <= y0 || x1 <= x0) return T(0);
+  // rows in parallel, summed in row order afterwards: the same result for
+  // any thread count
+  std::vector<T> rows(y1 - y0, T(0));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t r = 0; r < int64_t(y1 - y0); ++r) {
+    const uint32_t iy = y0 + uint32_t(r);
+    T acc = 0;
+    for (uint32_t ix = x0; ix < x1; ++ix) {
+      const uint32_t ipix = iy * img_w + ix;
+      const uint32_t tile = (iy / ts) * tw + ix / ts;
+      const uint32_t i0 = tile2idx[tile], num = tile2idx[tile + 1] - i0;
+      const PixelOut<T> o = raster_pixel(ix, iy, pixxydepth, conic, aabb, opacity, rgb, idx2pnt,
+                                         i0, num, frozen_nwalk[ipix], true, 0);
+      for (int k = 0; k < 3; ++k) {
+        const T d = o.c[k] - T(target[3 * ipix + k]);
+        acc += d * d;
+      }
+    }
+    rows[size_t(r)] = acc;
+  }
+  T loss = 0;
+  for (const T v : rows) loss += v;
+  return loss;
 }
 
 // ---------------------------------------------------------------------------
@@ -938,6 +1002,23 @@ void orc_gauss_rasterize_f32(uint32_t w, uint32_t h, const float* pixxydepth, co
                              float* pix2tlast, const uint32_t* frozen) {
   gauss_rasterize<float>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
                          pix2rgb, pix2alpha, pix2nwalk, pix2tlast, frozen);
+}
+void orc_gauss_rasterize_exp_ulp_f32(uint32_t w, uint32_t h, const float* pixxydepth,
+                                     const float* conic, const float* aabb, const float* opacity,
+                                     const float* rgb, uint32_t ts, const uint32_t* tile2idx,
+                                     const uint32_t* idx2pnt, float* pix2rgb, float* pix2alpha,
+                                     uint32_t* pix2nwalk, float* pix2tlast, int exp_ulp) {
+  gauss_rasterize<float>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
+                         pix2rgb, pix2alpha, pix2nwalk, pix2tlast, nullptr, exp_ulp);
+}
+double orc_gauss_l2_loss_tiles_f64(uint32_t w, uint32_t h, const double* pixxydepth,
+                                   const double* conic, const float* aabb, const double* opacity,
+                                   const double* rgb, uint32_t ts, const uint32_t* tile2idx,
+                                   const uint32_t* idx2pnt, const uint32_t* frozen,
+                                   const float* target, uint32_t tx0, uint32_t tx1, uint32_t ty0,
+                                   uint32_t ty1) {
+  return gauss_l2_loss_tiles<double>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
+                                     idx2pnt, frozen, target, tx0, tx1, ty0, ty1);
 }
 void orc_gauss_rasterize_f64(uint32_t w, uint32_t h, const double* pixxydepth,
                              const double* conic, const float* aabb, const double* opacity,

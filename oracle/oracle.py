@@ -53,6 +53,14 @@ def lib():
         L.orc_gauss_rasterize_f32.restype = None
         L.orc_gauss_rasterize_f32.argtypes = [_u32, _u32, _f32p, _f32p, _f32p, _f32p, _f32p,
                                               _u32, _u32p, _u32p, _f32p, _f32p, _u32p, _f32p, _vp]
+        L.orc_gauss_rasterize_exp_ulp_f32.restype = None
+        L.orc_gauss_rasterize_exp_ulp_f32.argtypes = [_u32, _u32, _f32p, _f32p, _f32p, _f32p,
+                                                      _f32p, _u32, _u32p, _u32p, _f32p, _f32p,
+                                                      _u32p, _f32p, ctypes.c_int]
+        L.orc_gauss_l2_loss_tiles_f64.restype = ctypes.c_double
+        L.orc_gauss_l2_loss_tiles_f64.argtypes = [_u32, _u32, _f64p, _f64p, _f32p, _f64p, _f64p,
+                                                  _u32, _u32p, _u32p, _u32p, _f32p, _u32, _u32,
+                                                  _u32, _u32]
         L.orc_gauss_rasterize_f64.restype = None
         L.orc_gauss_rasterize_f64.argtypes = [_u32, _u32, _f64p, _f64p, _f32p, _f64p, _f64p,
                                               _u32, _u32p, _u32p, _f64p, _f64p, _u32p, _f64p, _vp]
@@ -210,6 +218,59 @@ def gauss_rasterize(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2i
     f = lib().orc_gauss_rasterize_f32 if dtype == np.float32 else lib().orc_gauss_rasterize_f64
     f(w, h, pxd, cn, bb, op, col, ts, t2i, i2p, img, alpha, nwalk, tlast, _ptr(fz))
     return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tlast.reshape(h, w))
+
+
+def gauss_rasterize_exp_ulp(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
+                            idx2pnt, exp_ulp):
+    """gauss_rasterize (float) with every exp result moved exp_ulp ulps (a
+    planted perturbation: tests count the pixels whose termination it flips)."""
+    w, h = img_shape
+    pxd, cn, op, col = (_c(a, np.float32).reshape(-1) for a in (pixxydepth, conic, opacity, rgb))
+    bb = _c(aabb, np.float32).reshape(-1)
+    i2p = _c(idx2pnt, np.uint32)
+    if i2p.size == 0:
+        i2p = np.zeros(1, np.uint32)
+    img = np.zeros(h * w * 3, np.float32)
+    alpha = np.zeros(h * w, np.float32)
+    nwalk = np.zeros(h * w, np.uint32)
+    tlast = np.zeros(h * w, np.float32)
+    lib().orc_gauss_rasterize_exp_ulp_f32(w, h, pxd, cn, bb, op, col, ts,
+                                          _c(tile2idx, np.uint32), i2p, img, alpha, nwalk, tlast,
+                                          int(exp_ulp))
+    return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tlast.reshape(h, w))
+
+
+def gauss_l2_loss_tiles(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
+                        frozen_nwalk, target, tile_rect):
+    """float64: sum (rgb - target)^2 over the tiles tile_rect = (tx0, tx1, ty0,
+    ty1) (inclusive) of the tiled raster with every walk frozen at
+    frozen_nwalk. The arrays are passed as they are (no copies), so a caller
+    can change one Gaussian in place between calls: pixxydepth, conic,
+    opacity, rgb must be C-contiguous float64, aabb float32, the lists and
+    frozen_nwalk uint32, target float32."""
+    for a, dt in ((pixxydepth, np.float64), (conic, np.float64), (opacity, np.float64),
+                  (rgb, np.float64), (aabb, np.float32), (tile2idx, np.uint32),
+                  (idx2pnt, np.uint32), (frozen_nwalk, np.uint32), (target, np.float32)):
+        assert a.dtype == dt and a.flags.c_contiguous
+    w, h = img_shape
+    tx0, tx1, ty0, ty1 = (int(v) for v in tile_rect)
+    return float(lib().orc_gauss_l2_loss_tiles_f64(
+        w, h, pixxydepth.reshape(-1), conic.reshape(-1), aabb.reshape(-1), opacity.reshape(-1),
+        rgb.reshape(-1), ts, tile2idx, idx2pnt, frozen_nwalk.reshape(-1), target.reshape(-1),
+        tx0, tx1, ty0, ty1))
+
+
+def self_flips(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt, nwalk):
+    """Pixels whose walk length changes when every exp result of the oracle
+    moves one ulp up or one ulp down, against the oracle's own nwalk (the
+    union of both sets): the termination flips that the last bit of the
+    exponential decides."""
+    out = np.zeros(np.asarray(nwalk).shape, bool)
+    for u in (1, -1):
+        nw = gauss_rasterize_exp_ulp(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts,
+                                     tile2idx, idx2pnt, u)[2]
+        out |= nw != nwalk
+    return out
 
 
 def gauss_rasterize_bwd(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 import torch
 
-from _helpers import check_pixels, max_rel, u32
+from _helpers import check_grad, check_pixels, u32
 from del_splat_dlpack import scene
 
 pytestmark = pytest.mark.gpu
@@ -47,8 +47,10 @@ def test_plain_forward_100k_512(oracle):
     np.testing.assert_array_equal(u32(i2p), o_i2p)
     o_img, o_alpha, o_nwalk, _ = oracle.gauss_rasterize(shape, o_pxd, o_conic, o_aabb,
                                                         s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
+    own = oracle.self_flips(shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i,
+                            o_i2p, o_nwalk)
     flips = check_pixels(img.cpu().numpy(), alpha.cpu().numpy(), nwalk.cpu().numpy(), o_img,
-                         o_alpha, o_nwalk)
+                         o_alpha, o_nwalk, self_flip=own)
     print(f"pairs {len(o_i2p)}, termination flips {flips}", flush=True)
     assert o_alpha.max() > 0.99
 
@@ -112,6 +114,4 @@ def test_batched_adjoint_3m_8_views():
         ((img - targets[v]) ** 2).sum().backward()
     names = dict(xyz="xyz", scale="log_scale", quat="quat", opacity="opacity", rgb="rgb")
     for k, lk in names.items():
-        err = max_rel(g[k].cpu().numpy(), leaf[lk].grad.cpu().numpy())
-        print(f"{k}: {err:.2e}", flush=True)
-        assert err <= 1e-3, (k, err)
+        check_grad(g[k].cpu().numpy(), leaf[lk].grad.cpu().numpy(), k, max_rel_tol=1e-3)

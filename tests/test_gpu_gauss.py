@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from _helpers import check_pixels as _check_pixels, max_rel, small_scene, u32
+from _helpers import check_grad, check_pixels as _check_pixels, max_rel, small_scene, u32
 
 pytestmark = pytest.mark.gpu
 
@@ -312,9 +312,7 @@ def _grad_case(oracle, n, shape, seed, ts, with_alpha):
 def test_backward_matches_oracle_adjoint(oracle, ts, with_alpha):
     res = _grad_case(oracle, 3000, (160, 120), 20 + ts, ts, with_alpha)
     for name, (g, o) in res.items():
-        err = max_rel(g.cpu().numpy(), o)
-        print(f"{name}: max rel err {err:.2e}")
-        assert err <= 2e-3, (name, err)
+        check_grad(g.cpu().numpy(), o, name, abs_frac=1e-4)
 
 
 @pytest.mark.parametrize("ts,threads", [(16, 64), (16, 128), (16, 256), (32, 64), (32, 128),
@@ -342,9 +340,8 @@ def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
                                    u32(i2p), nwalk.cpu().numpy().view(np.uint32),
                                    tlast.cpu().numpy(), d_img, d_alpha)
     for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
-        err = max_rel(a.cpu().numpy(), b)
-        print(f"ts={ts} threads={threads} alpha={with_alpha} {name}: max rel err {err:.2e}")
-        assert err <= 2e-3, (name, err)
+        check_grad(a.cpu().numpy(), b, f"ts={ts} threads={threads} alpha={with_alpha} {name}",
+                   abs_frac=1e-4)
 
 
 @pytest.mark.parametrize("ts,threads", [(16, 0), (32, 512)])
@@ -444,10 +441,13 @@ def test_tile_schedule_orders_and_results():
 
 def test_headline_frame_matches_oracle(oracle):
     """The headline workload at full size (1M Gaussians, 1920x1080, 32x32
-    tiles, 512-thread forward and adjoint, walk-ordered tiles, fused L2
-    adjoint): projection and lists bit-exact, pixels within 1e-4 (termination
-    flips bounded), every gradient within 2e-3 of its largest magnitude, all
-    against the oracle's frame on the same inputs."""
+    tiles, 512-thread forward, walk-ordered tiles, fused L2 adjoint at the
+    benchmarked 256 threads and at 512): projection and lists bit-exact,
+    pixels within 1e-4 with termination flips bounded in size and no more
+    numerous than the oracle's own flips when its exp moves by one ulp,
+    every gradient per element (1e-2 relative above 1% of the largest
+    magnitude, 1e-4 of it below), all against the oracle's frame on the same
+    inputs."""
     import del_splat_dlpack.SplatGauss.torch as SG
     from del_splat_dlpack import scene
     shape, ts, th = (1920, 1080), 32, 512
@@ -463,9 +463,12 @@ def test_headline_frame_matches_oracle(oracle):
     cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
     img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, tile_cost=cost)
     order = SG.tile_order(cost)
-    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast, img,
-                                                _t(target), th, tile_order=order)
-    dxyz, dsc, dq = SG.project_bwd(xyz, sc, q, m16, shape, dpix, dcon, log_scale=False)
+    grads = {}
+    for thb in (256, 512):
+        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
+                                                    img, _t(target), thb, tile_order=order)
+        dxyz, dsc, dq = SG.project_bwd(xyz, sc, q, m16, shape, dpix, dcon, log_scale=False)
+        grads[thb] = [t.cpu().numpy() for t in (dxyz, dsc, dq, dop, drgb)]
     # the oracle's frame
     o_pxd, o_conic, o_aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
     np.testing.assert_array_equal(pxd.cpu().numpy(), o_pxd)
@@ -476,19 +479,21 @@ def test_headline_frame_matches_oracle(oracle):
     np.testing.assert_array_equal(u32(i2p), o_i2p)
     o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
+    own = oracle.self_flips(shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i,
+                            o_i2p, o_nwalk)
     flips = _check_pixels(img.cpu().numpy(), alpha.cpu().numpy(),
-                          nwalk.cpu().numpy().view(np.uint32), o_img, o_alpha, o_nwalk)
+                          nwalk.cpu().numpy().view(np.uint32), o_img, o_alpha, o_nwalk,
+                          self_flip=own)
     print(f"threshold flips: {flips}")
     o_dpix, o_dcon, o_dop, o_drgb = oracle.gauss_rasterize_bwd(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p, o_nwalk,
         o_tlast, (2.0 * (o_img - target)).astype(np.float32))
     o_dxyz, o_dsc, o_dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape,
                                                    o_dpix, o_dcon)
-    for name, g, o in (("xyz", dxyz, o_dxyz), ("scale", dsc, o_dsc), ("quat", dq, o_dq),
-                       ("opacity", dop, o_dop), ("rgb", drgb, o_drgb)):
-        err = max_rel(g.cpu().numpy().reshape(-1), np.asarray(o).reshape(-1))
-        print(f"{name}: max rel err {err:.2e}")
-        assert err <= 2e-3, (name, err)
+    for thb, gl in grads.items():
+        for name, g, o in zip(("xyz", "scale", "quat", "opacity", "rgb"), gl,
+                              (o_dxyz, o_dsc, o_dq, o_dop, o_drgb)):
+            check_grad(g, o, f"adjoint {thb} threads {name}", abs_frac=1e-4)
 
 
 def _crafted_boxes(n, shape, seed):
@@ -565,9 +570,7 @@ def test_box_edges_on_pixel_centres(oracle, ts, threads):
                                    nwalk.cpu().numpy().view(np.uint32), tlast.cpu().numpy(),
                                    d_img)
     for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
-        err = max_rel(a.cpu().numpy(), b)
-        print(f"ts={ts} threads={threads} {name}: max rel err {err:.2e}", flush=True)
-        assert err <= 2e-3, (name, err)
+        check_grad(a.cpu().numpy(), b, f"ts={ts} threads={threads} {name}", abs_frac=1e-4)
     # the crafted edges do decide pixels: moving every edge one ulp outward
     # changes the image
     wide = splat2.copy()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from _helpers import max_rel, u32
+from _helpers import check_grad, u32
 from del_splat_dlpack import _native, scene
 
 pytestmark = pytest.mark.gpu
@@ -87,9 +87,8 @@ def test_multiview_backward_matches_sum_of_per_view(ts, threads):
     total.backward()
     names = dict(xyz="xyz", scale="log_scale", quat="quat", opacity="opacity", rgb="rgb")
     for k, lk in names.items():
-        err = max_rel(g[k].cpu().numpy(), leaf[lk].grad.cpu().numpy())
-        print(f"{k}: {err:.2e}")
-        assert err <= 1e-3, (k, err)
+        check_grad(g[k].cpu().numpy(), leaf[lk].grad.cpu().numpy(), k, abs_frac=1e-4,
+                   max_rel_tol=1e-3)
     plan.close()
 
 
@@ -140,6 +139,47 @@ def test_changing_views_or_colour_drops_the_graph():
     plan.set_sh(0, 0)
     with pytest.raises(ValueError, match="nothing captured"):
         plan.replay()
+    plan.close()
+
+
+def test_backward_needs_a_forward_over_the_current_views():
+    """The adjoint reads the last forward's lists, walks and transmittances:
+    on a new plan, after set_views or set_sh, or after a capture that was not
+    replayed, the C ABI refuses it before launching anything (and so does the
+    Python plan); a forward, a training step or a replay makes it valid."""
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
+    s, views = _setup(n=3000, seed=46)
+    n = s["xyz"].shape[0]
+    args = [_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb")]
+    plan = MultiViewPlan(n, views, tile_size=16)
+    tgt = plan.new_image(3).zero_()
+    g = [torch.empty((n, c), device=DEV) for c in (3, 3, 4, 1, 3)]
+
+    def c_backward(out):
+        _native.call("dsplat_mv_backward", plan._plan, torch.cuda.current_stream().cuda_stream,
+                     *[a.data_ptr() for a in args[:4]], out.data_ptr(), None, tgt.data_ptr(),
+                     None, *[x.data_ptr() for x in g], _native.DSPLAT_SCALE_IS_LOG)
+
+    out = plan.new_image(3)
+    with pytest.raises(ValueError, match="no forward"):
+        c_backward(out)
+    with pytest.raises(ValueError, match="no forward"):
+        plan.backward(*args[:4], out_rgb=out, target=tgt)
+    out, _ = plan.forward(*args)
+    c_backward(out)
+    plan.set_views(views[::-1])
+    with pytest.raises(ValueError, match="no forward"):
+        c_backward(out)
+    o_rgb, o_a = plan.new_image(3), plan.new_image(1)
+    plan.capture_forward(*args, o_rgb, o_a)
+    with pytest.raises(ValueError, match="no forward"):
+        c_backward(o_rgb)  # captured, not run
+    plan.replay()
+    c_backward(o_rgb)
+    plan.set_sh(0, 0)
+    with pytest.raises(ValueError, match="no forward"):
+        plan.backward(*args[:4], out_rgb=o_rgb, target=tgt)
+    torch.cuda.synchronize()
     plan.close()
 
 

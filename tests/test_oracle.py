@@ -5,7 +5,7 @@ differences, and the reference's own (commented) known-answer test."""
 import numpy as np
 import pytest
 
-from _helpers import small_scene
+from _helpers import check_grad, small_scene
 from del_splat_dlpack import camera, scene
 
 
@@ -548,3 +548,72 @@ def test_adam_closed_form_and_torch(oracle):
         opt.step()
         oracle.adam(p, gs, m1, m2, 1e-3, 0.9, 0.999, 1e-8, step)
     np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------- planted exp perturbation, FD ---
+def test_exp_ulp_perturbation_and_self_flips(oracle):
+    """exp_ulp = 0 is the oracle's raster; moving every exp one ulp flips the
+    termination of a few pixels only, each at the 0.999 threshold, and moves
+    those pixels by less than the ~1e-3 of transmittance left there."""
+    shape, ts = (256, 192), 16
+    s, m = small_scene(20000, 70, shape)
+    pxd, conic, aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
+    t2i, i2p = oracle.tile2pnt_gauss(pxd, aabb, shape, ts)
+    img, alpha, nwalk, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"],
+                                                      s["rgb"], ts, t2i, i2p)
+    assert (alpha >= 0.999).mean() > 0.05
+    z = oracle.gauss_rasterize_exp_ulp(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i,
+                                       i2p, 0)
+    for a, b in zip(z, (img, alpha, nwalk, tlast)):
+        np.testing.assert_array_equal(a, b)
+    for u in (1, -1, 3):
+        img2, alpha2, nw2, _ = oracle.gauss_rasterize_exp_ulp(shape, pxd, conic, aabb,
+                                                              s["opacity"], s["rgb"], ts, t2i,
+                                                              i2p, u)
+        flip = nw2 != nwalk
+        assert flip.mean() < 1e-3
+        assert (np.abs(np.maximum(alpha, alpha2)[flip] - 0.999) < 2e-3).all()
+        assert np.abs(img2 - img).max() <= 1.1e-3
+        assert np.abs(img2 - img)[~flip].max() <= 1e-5
+        if u > 0:  # larger weights: at least as much alpha wherever the walk is the same
+            assert (alpha2[~flip] >= alpha[~flip] - 1e-6).all()
+    own = oracle.self_flips(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p, nwalk)
+    assert own.shape == nwalk.shape and own.sum() < 1e-3 * own.size
+
+
+def test_tile_loss_and_local_fd_match_the_adjoint(oracle):
+    """The tile-rectangle L2 loss over all tiles equals the full frozen-walk
+    frame's loss, and central differences through it (the full-frame FD
+    recipe of the GPU gradient test, _fd.fd_gradients) reproduce the oracle's
+    own fp64 adjoint chain for 12 Gaussians."""
+    from _fd import fd_gradients, visible_pick
+    f64 = np.float64
+    shape, ts = (200, 150), 16
+    s, m = small_scene(3000, 71, shape)
+    pxd32, conic32, aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
+    t2i, i2p = oracle.tile2pnt_gauss(pxd32, aabb, shape, ts)
+    _, _, nwalk, _ = oracle.gauss_rasterize(shape, pxd32, conic32, aabb, s["opacity"], s["rgb"],
+                                            ts, t2i, i2p)
+    target = scene.target_image(shape, 72)
+    pxd, conic, _, _ = oracle.gauss_project(s["xyz"].astype(f64), s["quat"].astype(f64),
+                                            np.exp(s["log_scale"].astype(f64)), m.astype(f64),
+                                            shape, dtype=f64)
+    img, alpha, _, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"], s["rgb"],
+                                                  ts, t2i, i2p, dtype=f64, frozen_nwalk=nwalk)
+    full = ((img - target.astype(f64)) ** 2).sum()
+    tw, th = (shape[0] + ts - 1) // ts, (shape[1] + ts - 1) // ts
+    tiles = oracle.gauss_l2_loss_tiles(shape, np.ascontiguousarray(pxd),
+                                       np.ascontiguousarray(conic), aabb,
+                                       s["opacity"].astype(f64), s["rgb"].astype(f64), ts, t2i,
+                                       i2p, nwalk.astype(np.uint32), target, (0, tw - 1, 0, th - 1))
+    assert abs(tiles - full) <= 1e-12 * full
+    dpix, dcon, dop, drgb = oracle.gauss_rasterize_bwd(shape, pxd, conic, aabb, s["opacity"],
+                                                       s["rgb"], ts, t2i, i2p, nwalk, tlast,
+                                                       2 * (img - target), dtype=f64)
+    dxyz, dls, dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], np.exp(s["log_scale"]), m,
+                                             shape, dpix, dcon, log_scale=True, dtype=f64)
+    pick = visible_pick(drgb, 12, 73)
+    fd = fd_gradients(oracle, s, m, shape, ts, t2i, i2p, aabb, nwalk, target, pick)
+    for name, an in (("xyz", dxyz), ("log_scale", dls), ("quat", dq), ("opacity", dop),
+                     ("rgb", drgb)):
+        check_grad(an[pick], fd[name], f"fd {name}")
