@@ -13,6 +13,8 @@
 // and a stable sort by tile id alone (bitwidth(T-1) bits instead of 64)
 // finishes the job: within a tile, pairs keep the (depth, index) order of the
 // first sort. Tile ranges are then the boundaries of the sorted tile ids.
+#include <mutex>
+
 #include "dsplat_common.hpp"
 #include "primitives.hpp"
 
@@ -438,6 +440,71 @@ extern "C" int dsplat_gauss_tile2idx_idx2pnt_ws(void* stream, uint32_t img_w, ui
                       GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)}, pnt2pixxydepth,
                       tile_size, tw, th, tile2idx, idx2pnt, capacity, workspace, workspace_bytes,
                       d_num_idx);
+}
+
+// ------------------------------------------------- allocation-free binning ---
+// The workspace of dsplat_gauss_tile2idx_idx2pnt_noalloc: a file-scope device
+// array of DSPLAT_NOALLOC_SCRATCH_BYTES, one per GPU (each GPU that loads the
+// library holds its copy until the process ends). Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, taken once per
+// device with that device current, and is kept per device.
+static __device__ __align__(256) unsigned char scratch[DSPLAT_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxScratchDevices] = {};
+}  // namespace
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  DSPLAT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "noalloc binning: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return dsplat_set_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                  "noalloc binning: hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return DSPLAT_OK;
+}
+
+extern "C" int dsplat_gauss_tile2idx_idx2pnt_noalloc(void* stream, uint32_t img_w,
+                                                     uint32_t img_h, uint32_t tile_size,
+                                                     uint32_t num_pnt,
+                                                     const float* pnt2pixxydepth,
+                                                     const float* pnt2pixaabb,
+                                                     uint32_t* tile2idx, uint32_t* idx2pnt,
+                                                     uint32_t capacity, uint32_t* d_num_idx) {
+  if (int rc = check_tiles(tile_size)) return rc;
+  if (!tile2idx || !d_num_idx || (num_pnt && (!pnt2pixxydepth || !pnt2pixaabb)) ||
+      (capacity && !idx2pnt))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "tile2idx_idx2pnt_noalloc: null buffer");
+  if (reinterpret_cast<uintptr_t>(pnt2pixaabb) & 15u)
+    return dsplat_set_error(DSPLAT_ERR_ARG,
+                            "tile2idx_idx2pnt_noalloc: aabb must be 16-byte aligned");
+  const size_t need = dsplat_tile2idx_idx2pnt_workspace_size(num_pnt, 0, capacity);
+  if (need > size_t(DSPLAT_NOALLOC_SCRATCH_BYTES))
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY,
+                            "tile2idx_idx2pnt_noalloc: this many Gaussians and pair capacity "
+                            "need more workspace than the static scratch holds "
+                            "(DSPLAT_NOALLOC_SCRATCH_BYTES); use dsplat_gauss_tile2idx_idx2pnt_ws");
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  uint32_t tw, th;
+  tile_grid(img_w, img_h, tile_size, &tw, &th);
+  // every kernel of the path initialises what it reads of the workspace, so
+  // the previous call's contents need no clearing
+  return bin_and_sort(as_stream(stream), num_pnt,
+                      GaussBoxes{reinterpret_cast<const float4*>(pnt2pixaabb)}, pnt2pixxydepth,
+                      tile_size, tw, th, tile2idx, idx2pnt, capacity, ws,
+                      size_t(DSPLAT_NOALLOC_SCRATCH_BYTES), d_num_idx);
 }
 
 extern "C" int dsplat_gauss_tile2idx_idx2pnt(void* stream, uint32_t img_w, uint32_t img_h,

@@ -21,6 +21,7 @@ _u64 = ctypes.c_uint64
 DSPLAT_SCALE_IS_LOG = 1
 DSPLAT_OPACITY_IS_LOGIT = 2
 DSPLAT_GRAD_OVERWRITE = 4
+DSPLAT_ERR_CAPACITY = -3  # status: workspace, scratch or capacity too small
 
 
 class ViewDesc(ctypes.Structure):
@@ -36,6 +37,7 @@ _SIGS = {
     "dsplat_gauss_count_pairs": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp]),
     "dsplat_tile2idx_idx2pnt_workspace_size": (_sz, [_u32, _u32, _u32]),
     "dsplat_gauss_tile2idx_idx2pnt_ws": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _sz, _vp]),
+    "dsplat_gauss_tile2idx_idx2pnt_noalloc": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp]),
     "dsplat_gauss_tile2idx_idx2pnt": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_u32)]),
     "dsplat_bin_points_workspace_size": (_sz, [_u32]),
     "dsplat_bin_pairs_workspace_size": (_sz, [_u32]),

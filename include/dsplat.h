@@ -88,6 +88,27 @@ int dsplat_gauss_tile2idx_idx2pnt_ws(void* stream, uint32_t img_w, uint32_t img_
                                      void* workspace, size_t workspace_bytes,
                                      uint32_t* d_num_idx);
 
+/* Size of the static device scratch of dsplat_gauss_tile2idx_idx2pnt_noalloc
+ * (128 MiB per GPU: e.g. 1M Gaussians with a pair capacity of 7M). */
+#define DSPLAT_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* dsplat_gauss_tile2idx_idx2pnt_ws without a workspace argument, allocating
+ * nothing: no hipMalloc, hipFree or allocating library call, for callers
+ * that cannot allocate (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid it). No counterpart in the reference. The workspace is
+ * a static device array of DSPLAT_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; calls whose dsplat_tile2idx_idx2pnt_workspace_size(num_pnt,
+ * -, capacity) exceeds it return DSPLAT_ERR_CAPACITY before any launch.
+ * Results are those of the _ws variant, whatever an earlier call left in the
+ * scratch. Every call on one GPU shares that GPU's scratch: calls that overlap
+ * on one GPU (other streams or host threads, not ordered on one stream) race
+ * and must not be issued. */
+int dsplat_gauss_tile2idx_idx2pnt_noalloc(void* stream, uint32_t img_w, uint32_t img_h,
+                                          uint32_t tile_size, uint32_t num_pnt,
+                                          const float* pnt2pixxydepth, const float* pnt2pixaabb,
+                                          uint32_t* tile2idx, uint32_t* idx2pnt,
+                                          uint32_t capacity, uint32_t* d_num_idx);
+
 /* The same binning split into phases, for callers that allocate idx2pnt
  * once K is known (the reference reads K back between counting and filling,
  * splat_gauss.rs:137-147), so the depth order can run while K travels to the
