@@ -34,11 +34,19 @@
 // safe_inverse_preserve_positive_definiteness, aabb2) are restated from their
 // mathematical definitions; their pins are closed-form tests in tests/.
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
+
+// Threads of the projection, binning, raster and adjoint loops (1: the
+// sequential restatement, the parity oracle; more: the all-core CPU baseline,
+// with identical results except the adjoint's float sums, which are reduced
+// in another order).
+static int g_threads = 1;
 
 namespace {
 
@@ -209,6 +217,8 @@ template <typename T>
 uint32_t gauss_project(uint32_t n, const T* xyz, const T* quat, const T* scale,
                        const T* m, uint32_t w, uint32_t h, T* pixxydepth, T* conic, T* aabb) {
   uint32_t nclamp = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nclamp) if (g_threads > 1) \
+    num_threads(g_threads)
   for (uint32_t i = 0; i < n; ++i) {
     Proj<T> p;
     project_one(xyz + 3 * i, quat + 4 * i, scale + 3 * i, m, w, h, p);
@@ -260,9 +270,15 @@ inline float depth_plus_one(float ndcz) {
 // tile2pnt (tile_acceleration.rs:3-58) over explicit boxes. Writes tile2idx
 // [T+1]; if idx2pnt != null also the stably sorted pnt list and, if keys !=
 // null, the sorted 64-bit keys tile<<32 | bits(depth+1).
+uint64_t tile2pnt_parallel(uint32_t n, const float* boxes, const float* pixxydepth,
+                           uint32_t img_w, uint32_t img_h, uint32_t ts, uint32_t* tile2idx,
+                           uint32_t* idx2pnt, uint64_t* keys);
+
 uint64_t tile2pnt(uint32_t n, const float* boxes, const float* pixxydepth,
                   uint32_t img_w, uint32_t img_h, uint32_t ts,
                   uint32_t* tile2idx, uint32_t* idx2pnt, uint64_t* keys) {
+  if (g_threads > 1)
+    return tile2pnt_parallel(n, boxes, pixxydepth, img_w, img_h, ts, tile2idx, idx2pnt, keys);
   const uint32_t tw = (img_w + ts - 1) / ts, th = (img_h + ts - 1) / ts;
   const uint32_t nt = tw * th;
   std::vector<uint64_t> cnt(nt + 1, 0);
@@ -292,6 +308,79 @@ uint64_t tile2pnt(uint32_t n, const float* boxes, const float* pixxydepth,
   for (uint64_t k = 0; k < K; ++k) {
     idx2pnt[k] = ents[k].pnt;
     if (keys) keys[k] = (uint64_t(ents[k].tile) << 32) | f32_bits(ents[k].depth);
+  }
+  return K;
+}
+
+// The same result with g_threads threads: a stable counting sort by tile
+// (each thread fills the pairs of a contiguous index range into its own slots
+// of every tile, so a tile's entries stay in index order), then a stable sort
+// of each tile's entries by depth -- the order of the sequential stable sort
+// by (tile, depth).
+uint64_t tile2pnt_parallel(uint32_t n, const float* boxes, const float* pixxydepth,
+                           uint32_t img_w, uint32_t img_h, uint32_t ts, uint32_t* tile2idx,
+                           uint32_t* idx2pnt, uint64_t* keys) {
+  const uint32_t tw = (img_w + ts - 1) / ts, th = (img_h + ts - 1) / ts;
+  const uint32_t nt = tw * th;
+  const int nthr = g_threads;
+  std::vector<uint64_t> cnt(size_t(nthr) * nt, 0);  // [thread][tile]
+  auto range = [&](int t, uint32_t* a, uint32_t* b) {
+    *a = uint32_t(uint64_t(n) * t / nthr);
+    *b = uint32_t(uint64_t(n) * (t + 1) / nthr);
+  };
+#pragma omp parallel num_threads(nthr)
+  {
+    const int t = omp_get_thread_num();
+    uint32_t a, b;
+    range(t, &a, &b);
+    uint64_t* c = cnt.data() + size_t(t) * nt;
+    for (uint32_t i = a; i < b; ++i) {
+      TileRect r = tile_rect(boxes + 4 * i, ts, tw, th);
+      for (int ix = r.x0; ix < r.x1; ++ix)
+        for (int iy = r.y0; iy < r.y1; ++iy) c[iy * tw + ix] += 1;
+    }
+  }
+  std::vector<uint64_t> tbase(nt + 1, 0);
+  for (uint32_t tile = 0; tile < nt; ++tile) {
+    uint64_t sum = 0;
+    for (int t = 0; t < nthr; ++t) sum += cnt[size_t(t) * nt + tile];
+    tbase[tile + 1] = tbase[tile] + sum;
+  }
+  const uint64_t K = tbase[nt];
+  if (tile2idx) for (uint32_t t = 0; t <= nt; ++t) tile2idx[t] = uint32_t(tbase[t]);
+  if (!idx2pnt) return K;
+  // per (thread, tile) first slot
+  for (uint32_t tile = 0; tile < nt; ++tile) {
+    uint64_t run = tbase[tile];
+    for (int t = 0; t < nthr; ++t) {
+      const uint64_t v = cnt[size_t(t) * nt + tile];
+      cnt[size_t(t) * nt + tile] = run;
+      run += v;
+    }
+  }
+  struct Ent { uint32_t pnt; float depth; };
+  std::vector<Ent> ents(K);
+#pragma omp parallel num_threads(nthr)
+  {
+    const int t = omp_get_thread_num();
+    uint32_t a, b;
+    range(t, &a, &b);
+    uint64_t* c = cnt.data() + size_t(t) * nt;
+    for (uint32_t i = a; i < b; ++i) {
+      TileRect r = tile_rect(boxes + 4 * i, ts, tw, th);
+      const float d = depth_plus_one(pixxydepth[3 * i + 2]);
+      for (int ix = r.x0; ix < r.x1; ++ix)
+        for (int iy = r.y0; iy < r.y1; ++iy) ents[c[iy * tw + ix]++] = {i, d};
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthr)
+  for (int64_t tile = 0; tile < int64_t(nt); ++tile) {
+    std::stable_sort(ents.begin() + int64_t(tbase[tile]), ents.begin() + int64_t(tbase[tile + 1]),
+                     [](const Ent& x, const Ent& y) { return x.depth < y.depth; });
+    for (uint64_t k = tbase[tile]; k < tbase[tile + 1]; ++k) {
+      idx2pnt[k] = ents[k].pnt;
+      if (keys) keys[k] = (uint64_t(tile) << 32) | f32_bits(ents[k].depth);
+    }
   }
   return K;
 }
@@ -361,6 +450,7 @@ void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const 
                      T* pix2rgb, T* pix2alpha, uint32_t* pix2nwalk, T* pix2tlast,
                      const uint32_t* frozen_nwalk, int exp_ulp = 0) {
   const uint32_t tw = (img_w + ts - 1) / ts;
+#pragma omp parallel for schedule(dynamic, 2) if (g_threads > 1) num_threads(g_threads)
   for (uint32_t iy = 0; iy < img_h; ++iy) {
     for (uint32_t ix = 0; ix < img_w; ++ix) {
       const uint32_t ipix = iy * img_w + ix;
@@ -430,9 +520,10 @@ void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, co
                          const uint32_t* tile2idx, const uint32_t* idx2pnt,
                          const uint32_t* pix2nwalk, const T* pix2tlast,
                          const T* d_pix2rgb, const T* d_pix2alpha,
-                         T* d_pixxy, T* d_conic, T* d_opacity, T* d_rgb) {
+                         T* d_pixxy, T* d_conic, T* d_opacity, T* d_rgb,
+                         uint32_t row0 = 0, uint32_t row1 = ~0u) {
   const uint32_t tw = (img_w + ts - 1) / ts;
-  for (uint32_t iy = 0; iy < img_h; ++iy) {
+  for (uint32_t iy = row0; iy < img_h && iy < row1; ++iy) {
     for (uint32_t ix = 0; ix < img_w; ++ix) {
       const uint32_t ipix = iy * img_w + ix;
       const uint32_t tile = (iy / ts) * tw + ix / ts;
@@ -481,6 +572,40 @@ void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, co
   }
 }
 
+// The adjoint with g_threads threads over groups of rows, each thread into its
+// own gradient buffers (n Gaussians), summed in thread order afterwards.
+template <typename T>
+void gauss_rasterize_bwd_mt(uint32_t n, uint32_t img_w, uint32_t img_h, const T* pixxydepth,
+                            const T* conic, const float* aabb, const T* opacity, const T* rgb,
+                            uint32_t ts, const uint32_t* tile2idx, const uint32_t* idx2pnt,
+                            const uint32_t* pix2nwalk, const T* pix2tlast, const T* d_pix2rgb,
+                            const T* d_pix2alpha, T* d_pixxy, T* d_conic, T* d_opacity,
+                            T* d_rgb) {
+  const int nthr = g_threads;
+  std::vector<T> buf(size_t(nthr) * 9 * n, T(0));
+#pragma omp parallel num_threads(nthr)
+  {
+    const int t = omp_get_thread_num();
+    T* b = buf.data() + size_t(t) * 9 * n;
+    // groups of 4 rows dealt round-robin (a fixed assignment: the same sums
+    // on every run; interleaved, as the image centre is the heavy part)
+    for (uint32_t r0 = 4u * uint32_t(t); r0 < img_h; r0 += 4u * uint32_t(nthr))
+      gauss_rasterize_bwd<T>(img_w, img_h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
+                             idx2pnt, pix2nwalk, pix2tlast, d_pix2rgb, d_pix2alpha, b, b + 2 * n,
+                             b + 5 * n, b + 6 * n, r0, r0 + 4u);
+  }
+#pragma omp parallel for schedule(static) num_threads(nthr)
+  for (int64_t i = 0; i < int64_t(9) * n; ++i) {
+    T v = 0;
+    for (int t = 0; t < nthr; ++t) v += buf[size_t(t) * 9 * n + size_t(i)];
+    T* dst = i < 2 * int64_t(n) ? d_pixxy + i
+             : i < 5 * int64_t(n) ? d_conic + (i - 2 * int64_t(n))
+             : i < 6 * int64_t(n) ? d_opacity + (i - 5 * int64_t(n))
+                                  : d_rgb + (i - 6 * int64_t(n));
+    *dst += v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint of the projection (new). Inputs: 2D grads from the raster adjoint.
 // Outputs (overwritten): d_xyz[N,3], d_scale[N,3] (w.r.t. sigma, or w.r.t.
@@ -492,6 +617,7 @@ void gauss_project_bwd(uint32_t n, const T* xyz, const T* quat, const T* scale, 
                        T* d_xyz, T* d_scale, T* d_quat, int log_scale) {
   T a[6];
   ndc2pix(img_w, img_h, a);
+#pragma omp parallel for schedule(static) if (g_threads > 1) num_threads(g_threads)
   for (uint32_t i = 0; i < n; ++i) {
     Proj<T> p;
     project_one(xyz + 3 * i, quat + 4 * i, scale + 3 * i, m, img_w, img_h, p);
@@ -1040,6 +1166,26 @@ void orc_gauss_rasterize_bwd_f32(uint32_t w, uint32_t h, const float* pixxydepth
                              nwalk, tlast, d_rgbimg, d_alphaimg, d_pixxy, d_conic, d_opacity,
                              d_rgb);
 }
+// Threads of the CPU loops (1 = the sequential parity oracle); returns the
+// number set, clamped to [1, omp_get_max_threads()].
+int orc_set_threads(int n) {
+  const int mx = omp_get_max_threads();
+  g_threads = n < 1 ? 1 : (n > mx ? mx : n);
+  return g_threads;
+}
+
+void orc_gauss_rasterize_bwd_mt_f32(uint32_t n, uint32_t w, uint32_t h, const float* pixxydepth,
+                                    const float* conic, const float* aabb, const float* opacity,
+                                    const float* rgb, uint32_t ts, const uint32_t* tile2idx,
+                                    const uint32_t* idx2pnt, const uint32_t* nwalk,
+                                    const float* tlast, const float* d_rgbimg,
+                                    const float* d_alphaimg, float* d_pixxy, float* d_conic,
+                                    float* d_opacity, float* d_rgb) {
+  gauss_rasterize_bwd_mt<float>(n, w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
+                                idx2pnt, nwalk, tlast, d_rgbimg, d_alphaimg, d_pixxy, d_conic,
+                                d_opacity, d_rgb);
+}
+
 void orc_gauss_rasterize_bwd_f64(uint32_t w, uint32_t h, const double* pixxydepth,
                                  const double* conic, const float* aabb, const double* opacity,
                                  const double* rgb, uint32_t ts, const uint32_t* tile2idx,

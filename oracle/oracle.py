@@ -68,6 +68,13 @@ def lib():
         L.orc_gauss_rasterize_bwd_f32.argtypes = [_u32, _u32, _f32p, _f32p, _f32p, _f32p, _f32p,
                                                   _u32, _u32p, _u32p, _u32p, _f32p, _f32p, _vp,
                                                   _f32p, _f32p, _f32p, _f32p]
+        L.orc_set_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_gauss_rasterize_bwd_mt_f32.restype = None
+        L.orc_gauss_rasterize_bwd_mt_f32.argtypes = [_u32, _u32, _u32, _f32p, _f32p, _f32p,
+                                                     _f32p, _f32p, _u32, _u32p, _u32p, _u32p,
+                                                     _f32p, _f32p, _vp, _f32p, _f32p, _f32p,
+                                                     _f32p]
         L.orc_gauss_rasterize_bwd_f64.restype = None
         L.orc_gauss_rasterize_bwd_f64.argtypes = [_u32, _u32, _f64p, _f64p, _f32p, _f64p, _f64p,
                                                   _u32, _u32p, _u32p, _u32p, _f64p, _f64p, _vp,
@@ -293,6 +300,34 @@ def gauss_rasterize_bwd(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, ti
     f(w, h, pxd, cn, bb, op, col, ts, _c(tile2idx, np.uint32), i2p,
       _c(nwalk, np.uint32).reshape(-1), tl, dimg, _ptr(da), d_pixxy, d_conic, d_op, d_rgb)
     return d_pixxy.reshape(n, 2), d_conic.reshape(n, 3), d_op, d_rgb.reshape(n, 3)
+
+
+def set_threads(n):
+    """Threads of the oracle's loops: 1 (the default) is the sequential parity
+    oracle; more give the all-core CPU baseline (same results, except the
+    adjoint's float sums in another order). Returns the number set."""
+    return int(lib().orc_set_threads(int(n)))
+
+
+def gauss_rasterize_bwd_mt(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
+                           idx2pnt, nwalk, tlast, d_img, d_alpha=None):
+    """gauss_rasterize_bwd (float) over the threads set by set_threads (row
+    bands into per-thread gradient buffers)."""
+    w, h = img_shape
+    pxd, cn, op, col, tl, dimg = (_c(a, np.float32).reshape(-1)
+                                  for a in (pixxydepth, conic, opacity, rgb, tlast, d_img))
+    bb = _c(aabb, np.float32).reshape(-1)
+    n = op.size
+    da = None if d_alpha is None else _c(d_alpha, np.float32).reshape(-1)
+    i2p = _c(idx2pnt, np.uint32)
+    if i2p.size == 0:
+        i2p = np.zeros(1, np.uint32)
+    out = [np.zeros(k * n, np.float32) for k in (2, 3, 1, 3)]
+    lib().orc_gauss_rasterize_bwd_mt_f32(n, w, h, pxd, cn, bb, op, col, ts,
+                                         _c(tile2idx, np.uint32), i2p,
+                                         _c(nwalk, np.uint32).reshape(-1), tl, dimg, _ptr(da),
+                                         *out)
+    return out[0].reshape(n, 2), out[1].reshape(n, 3), out[2], out[3].reshape(n, 3)
 
 
 def gauss_project_bwd(xyz, quat, scale, world2ndc, img_shape, d_pixxy, d_conic, log_scale=False,

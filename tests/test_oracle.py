@@ -617,3 +617,36 @@ def test_tile_loss_and_local_fd_match_the_adjoint(oracle):
     for name, an in (("xyz", dxyz), ("log_scale", dls), ("quat", dq), ("opacity", dop),
                      ("rgb", drgb)):
         check_grad(an[pick], fd[name], f"fd {name}")
+
+
+def test_multithreaded_oracle_matches_sequential(oracle):
+    """The all-core CPU baseline (oracle.set_threads > 1) computes the same
+    frame as the sequential parity oracle: projection, binning and raster
+    bit-exact (the binning as a per-thread stable counting sort by tile, then
+    per-tile depth sorts), adjoints equal up to the order of float sums."""
+    import os
+    shape, ts = (240, 176), 16
+    s, m = small_scene(20000, 74, shape)
+    ref = {}
+    for threads in (1, min(4, os.cpu_count() or 1)):
+        assert oracle.set_threads(threads) >= 1
+        try:
+            pxd, conic, aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
+            t2i, i2p, keys = oracle.tile2pnt_gauss(pxd, aabb, shape, ts, with_keys=True)
+            img, alpha, nwalk, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb,
+                                                              s["opacity"], s["rgb"], ts, t2i, i2p)
+            d_img = (2 * img - 1).astype(np.float32)
+            fn = oracle.gauss_rasterize_bwd if threads == 1 else oracle.gauss_rasterize_bwd_mt
+            d = fn(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p, nwalk, tlast,
+                   d_img)
+            g = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape, d[0], d[1])
+        finally:
+            oracle.set_threads(1)
+        out = [pxd, conic, aabb, t2i, i2p, keys, img, alpha, nwalk, tlast] + list(d) + list(g)
+        if threads == 1:
+            ref = out
+            continue
+        for a, b in zip(out[:10], ref[:10]):
+            np.testing.assert_array_equal(a, b)
+        for a, b in zip(out[10:], ref[10:]):
+            check_grad(a, b, "multithreaded adjoint", rel=1e-4, abs_frac=1e-6)
