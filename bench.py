@@ -19,6 +19,17 @@ tile of the batched path come from the autotuner.
 Training section: 3M Gaussians, 8 uneven views drawn afresh per step, fused
 forward + L2 adjoint + Adam on all parameters.
 
+Forward section: 100k Gaussians in the unit ball, 512x512, 16x16 tiles:
+projection, binning and the tiled forward raster (RGB + alpha).
+
+Z-buffer section: 2M points uniform in the unit cube, per-point pixel radius
+U[0.5,3], 1920x1080: point projection and the circle z-buffer (packed
+depth/index atomicMin per covered pixel, then the colour gather).
+
+CPU baseline: the oracle (a C++ restatement of del-splat-cpu) per stage,
+median of 5 runs, single-threaded (as the reference's CPU code runs) and with
+OpenMP over the host cores.
+
 Prints one JSON line.
   python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu-baseline] [--no-batched]
                   [--no-train]
@@ -33,12 +44,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "del-splat_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# VALU lane operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
-# (a wave64 instruction takes 4 cycles; packed FP32 is not used)
-VALU_PEAK_TOPS = 39.32
-# operations per (pixel, list entry) evaluation of the adjoint, the survey's
-# model (~20 forward, 2.5x that backward)
-VALU_OPS_BWD = 50
 
 
 def parse():
@@ -59,6 +64,9 @@ def parse():
                     help="bin each frame with the K read-back (the reference's tile2idx_idx2pnt "
                          "semantics) instead of the no-sync capacity path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sections", action="store_true",
+                    help="skip the forward (100k, 512x512) and z-buffer (2M points) sections")
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--train-gauss", type=int, default=3_000_000)
@@ -80,27 +88,194 @@ def copy_calibration(mb):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
 
 
-def cpu_baseline(s, m, shape, ts):
-    """The oracle (C++ restatement of del-splat-cpu) on one full frame,
-    single-threaded: projection, binning + stable sort, tiled raster, adjoint."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s, m, shape, ts, reps):
+    """The oracle (C++ restatement of del-splat-cpu) on the headline frame and
+    on the 100k-Gaussian 512x512 forward: each stage's median over `reps`
+    runs, single-threaded (the reference's CPU path has no threading) and with
+    OpenMP over the host cores (the threads the process may use)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
     from del_splat_dlpack import scene
     tgt = scene.target_image(shape, 102)
-    t0 = time.perf_counter()
-    pxd, conic, aabb, _ = O.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
-    t2i, i2p = O.tile2pnt_gauss(pxd, aabb, shape, ts)
-    img, alpha, nwalk, tlast = O.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"], s["rgb"],
-                                                 ts, t2i, i2p)
-    d_img = (2.0 * (img - tgt)).astype(np.float32)
-    d = O.gauss_rasterize_bwd(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p,
-                              nwalk, tlast, d_img)
-    O.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape, d[0], d[1], log_scale=True)
-    dt = time.perf_counter() - t0
-    return {"value": round(dt * 1000.0, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+    s0 = scene.gaussians(100_000, 0)
+    shape0 = (512, 512)
+    m0 = scene.front_camera(shape0)
+
+    def frame(S, M, shp, t, threads, adjoint):
+        st = {}
+        t0 = time.perf_counter()
+        pxd, conic, aabb, _ = O.gauss_project(S["xyz"], S["quat"], S["scale"], M, shp)
+        t1 = time.perf_counter()
+        t2i, i2p = O.tile2pnt_gauss(pxd, aabb, shp, t)
+        t2 = time.perf_counter()
+        img, alpha, nwalk, tlast = O.gauss_rasterize(shp, pxd, conic, aabb, S["opacity"],
+                                                     S["rgb"], t, t2i, i2p)
+        t3 = time.perf_counter()
+        st.update(project=t1 - t0, bin=t2 - t1, raster=t3 - t2)
+        if adjoint:
+            d_img = (2.0 * (img - tgt)).astype(np.float32)
+            f = O.gauss_rasterize_bwd if threads == 1 else O.gauss_rasterize_bwd_mt
+            d = f(shp, pxd, conic, aabb, S["opacity"], S["rgb"], t, t2i, i2p, nwalk, tlast,
+                  d_img)
+            O.gauss_project_bwd(S["xyz"], S["quat"], S["scale"], M, shp, d[0], d[1],
+                                log_scale=True)
+            st["adjoint"] = time.perf_counter() - t3
+        return st
+
+    def medians(threads, S, M, shp, t, adjoint):
+        runs = [frame(S, M, shp, t, threads, adjoint) for _ in range(reps)]
+        med = {k: 1000.0 * float(np.median([r[k] for r in runs])) for k in runs[0]}
+        return {k: round(v, 1) for k, v in med.items()}, round(sum(med.values()), 1)
+
+    out = {}
+    try:
+        for label, want in (("single", 1), ("all_cores", os.cpu_count() or 1)):
+            threads = O.set_threads(want)
+            st2, tot2 = medians(threads, s, m, shape, ts, True)
+            st0, tot0 = medians(threads, s0, m0, shape0, 16, False)
+            out[label] = {"threads": threads, "frame_ms": tot2, "stages_ms": st2,
+                          "forward_100k_512_ms": tot0, "forward_100k_512_stages_ms": st0}
+    finally:
+        O.set_threads(1)
+    one, allc = out["single"], out["all_cores"]
+    return {"value": one["frame_ms"], "unit": "ms/frame", "cores": 1, "kind": "port",
             "sample": f"one full frame ({s['xyz'].shape[0]} Gaussians, {shape[0]}x{shape[1]}, "
-                      "fwd+bwd) of the same workload, oracle C++ -O3, 1 thread"}
+                      f"fwd+bwd) of the same workload, per-stage medians of {reps} runs, oracle "
+                      "C++ -O3, 1 thread",
+            "stages_ms": one["stages_ms"],
+            "forward_100k_512": {"value": one["forward_100k_512_ms"], "unit": "ms/frame",
+                                 "stages_ms": one["forward_100k_512_stages_ms"]},
+            "all_cores": {"value": allc["frame_ms"], "unit": "ms/frame", "cores": allc["threads"],
+                          "stages_ms": allc["stages_ms"],
+                          "forward_100k_512": allc["forward_100k_512_ms"]},
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def section_forward(args, torch, np, SG, scene):
+    """The plain forward of 100k Gaussians at 512x512 with 16x16 tiles
+    (projection, binning without a host wait, forward raster with RGB and
+    alpha), one frame per step; the raster kernel bracketed by HIP events on
+    the library's stream (torch's current stream)."""
+    dev = torch.device("cuda:0")
+    shape, ts, n = (512, 512), 16, 100_000
+    s = scene.gaussians(n, 0)
+    m = scene.front_camera(shape)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    m16 = t(m)
+    pxd0, _, aabb0, _ = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
+    k0 = SG.bin_gauss(shape, ts, pxd0, aabb0)[1].numel()
+    cap = k0 + k0 // 8 + 65536
+    steps = max(args.steps, 20)
+    d_k = [torch.zeros(1, device=dev, dtype=torch.int32) for _ in range(steps)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+
+    def step(i, ev=None):
+        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
+        t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, d_k[i])
+        if ev:
+            ev[0].record()
+        out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
+        if ev:
+            ev[1].record()
+        return out
+
+    for _ in range(args.warmup):
+        step(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        img, alpha, nwalk, _ = step(i, evs[i])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    ks = [int(k.item()) for k in d_k]
+    if max(ks) > cap:
+        raise SystemExit("bench: forward section overflowed its pair capacity")
+    fwd_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    K, P = ks[-1], shape[0] * shape[1]
+    model = 40 * K + 24 * P  # records of the list entries + RGBA, T, n_contrib
+    ach = model / (fwd_ms * 1e-3) / 1e9
+    return {"workload": f"3dgs forward, {n} Gaussians, {shape[0]}x{shape[1]}, tile {ts}",
+            "value": round(ms, 4), "unit": "ms/frame", "steps": steps, "pairs_K": K,
+            "roofline": {"bound": "hbm", "kernel": "raster_fwd", "achieved": round(ach, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "kernel_ms": round(fwd_ms, 4), "bytes": model}}
+
+
+def section_zbuffer(args, torch, np, scene):
+    """2M points uniform in the unit cube, pixel radius U[0.5,3], RGB U[0,1],
+    1920x1080: the point projection and the circle z-buffer (clear, packed
+    depth/index atomicMin over every covered pixel, colour gather), one frame
+    per step; the z-buffer call bracketed by HIP events."""
+    from del_splat_dlpack import _native
+    from del_splat_dlpack._torch_util import ptr, stream_ptr
+    dev = torch.device("cuda:0")
+    shape, n = (1920, 1080), 2_000_000
+    p = scene.points(n, 1)
+    m = scene.front_camera(shape)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    xyz, rad, rgb, m16 = t(p["xyz"]), t(p["pixrad"]), t(p["rgb"]), t(m)
+    w, h = shape
+    pxz = torch.empty((n, 3), device=dev)
+    prad = torch.empty(n, device=dev)
+    img = torch.empty((h, w, 3), device=dev)
+    dep = torch.empty((h, w), device=dev)
+    keys = torch.empty(h * w, device=dev, dtype=torch.int64)
+    steps = max(args.steps, 20)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+
+    def step(ev=None):
+        st = stream_ptr(dev)
+        _native.call("dsplat_circle_pnt2splat3_to_pnt2splat2", st, n, ptr(xyz), ptr(pxz),
+                     ptr(prad), ptr(m16), w, h, 0.0)
+        if ev:
+            ev[0].record()
+        _native.call("dsplat_circle_rasterize_zbuffer", st, n, ptr(pxz), ptr(rad), ptr(rgb),
+                     ptr(dep), ptr(img), w, h, ptr(keys))
+        if ev:
+            ev[1].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    zb_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    # pixel centres within a disc of radius r: pi r^2 expected; counted over
+    # the points whose depth is inside [-1, 1] and whose disc meets the image
+    z = pxz[:, 2]
+    x, y = pxz[:, 0], pxz[:, 1]
+    on = (z >= -1) & (z <= 1) & (x + rad >= 0) & (x - rad <= w) & (y + rad >= 0) & (y - rad <= h)
+    hits = float((np.pi * rad[on] ** 2).sum().item())
+    P = w * h
+    covered = int((dep < 3.0e38).sum().item())
+    # clear 8P, points 16N, one 8-byte atomic per hit, resolve 8P keys +
+    # 12 B colour per covered pixel + 16P out
+    model = 8 * P + 16 * n + 8 * hits + 8 * P + 12 * covered + 16 * P
+    ach = model / (zb_ms * 1e-3) / 1e9
+    return {"workload": f"circle z-buffer, {n} points, {w}x{h}, pixel radius U[0.5,3]",
+            "value": round(ms, 4), "unit": "ms/frame", "steps": steps,
+            "covered_pixels": covered, "hits_expected": round(hits),
+            "roofline": {"bound": "hbm", "kernel": "zbuffer (clear + atomicMin + gather)",
+                         "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "kernel_ms": round(zb_ms, 4),
+                         "bytes": int(model)}}
 
 
 def headline(args, torch, np, SG, scene):
@@ -391,6 +566,18 @@ def train(args, torch, np, SG, scene):
                     "image generation and upload)"}
 
 
+def _profile_entry(name, dom, args):
+    """The entry of profiles/<name> for the dominant kernel at this schedule
+    (keyed by kernel, tile size and workgroup size: the counters depend on
+    the schedule), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    th = {"raster_fwd": args.threads, "raster_bwd": args.threads_bwd}.get(dom, 0)
+    try:
+        return json.load(open(path)).get(f"{dom}@tile{args.tile}/{th}")
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -420,18 +607,9 @@ def main():
     # adjoint (the kernel the timed steps bracket with events)
     dom_ms = stage_ms["raster_bwd_timed"] if dom == "raster_bwd" else stage_ms[dom]
     achieved = model[dom] / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs,
-        # keyed by kernel, tile size and workgroup size (the traffic depends on
-        # the schedule)
-        th = {"raster_fwd": args.threads, "raster_bwd": args.threads_bwd}.get(dom, 0)
-        try:
-            entry = json.load(open(tpath)).get(f"{dom}@tile{args.tile}/{th}")
-            traffic = None if entry is None else entry["bytes"]
-        except Exception:
-            traffic = None
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+    entry = _profile_entry("traffic.json", dom, args)
+    traffic = None if entry is None else entry["bytes"]
     out = {
         "metric": "fwd+bwd ms/frame @1M Gaussians 1080p",
         "value": round(ms, 3),
@@ -449,7 +627,8 @@ def main():
                                f"{args.tile}", "num_gauss": N, "width": shape[0],
                    "height": shape[1], "tile": args.tile, "threads_fwd": args.threads,
                    "threads_bwd": args.threads_bwd,
-                   "pairs_K": K, "parallelism": "single GPU",
+                   "pairs_K": K, "pixel_entry_evaluations": evals,
+                   "parallelism": "single GPU",
                    "binning": args.binning,
                    "autotune_ms": {k: round(v, 3) for k, v in
                                    (getattr(args, "autotune", None) or {}).items()}},
@@ -457,21 +636,22 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        # the raster kernels issue VALU instructions back to back: the same
-        # kernel against the VALU peak (one op per lane per cycle, no packed
-        # FP32) in algorithmic terms, E evaluations x ops per evaluation
-        "valu": {"kernel": "raster_bwd", "evaluations": evals, "ops_per_eval": VALU_OPS_BWD,
-                 "achieved": round(evals * VALU_OPS_BWD / (stage_ms["raster_bwd_timed"] * 1e-3) / 1e12, 2),
-                 "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
-                 "frac": round(evals * VALU_OPS_BWD / (stage_ms["raster_bwd_timed"] * 1e-3) / 1e12
-                               / VALU_PEAK_TOPS, 4)},
     }
+    sq = _profile_entry("sq.json", dom, args)
+    if sq is not None:
+        # the dominant kernel's VALU busy ratio from the committed SQ counter
+        # run of this schedule: SQ_ACTIVE_INST_VALU x 4 cycles over the SIMDs'
+        # cycles in the kernel's duration
+        out["valu"] = dict(sq, kernel=dom)
+    if not args.no_sections:
+        out["forward_100k_512"] = section_forward(args, torch, np, SG, scene)
+        out["zbuffer_2m_1080p"] = section_zbuffer(args, torch, np, scene)
     if not args.no_batched:
         out["batched"] = batched(args, torch, np, SG, scene)
     if not args.no_train:
         out["train"] = train(args, torch, np, SG, scene)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(s, m, shape, args.tile)
+        out["cpu_baseline"] = cpu_baseline(s, m, shape, args.tile, args.cpu_reps)
     print(json.dumps(out), flush=True)
 
 
