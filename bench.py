@@ -297,7 +297,7 @@ def headline(args, torch, np, SG, scene):
             args.num_gauss, m16, shape, (xyz, ls, q, op, rgb), target, use_cache=False)
         args.tile = ts
         args.autotune = tune
-    stages = ["project", "bin", "raster_fwd", "loss_grad", "raster_bwd", "project_bwd"]
+    stages = ["project", "bin", "raster_fwd", "raster_bwd", "project_bwd"]
     # tile schedule: each frame's forward is ordered by the previous frame's
     # per-tile walks (a hint; raster order on the first frame), its adjoint by
     # its own; results do not depend on the order
@@ -336,7 +336,11 @@ def headline(args, torch, np, SG, scene):
             if ev is not None and name in ev:
                 ev[name][0 if first else 1].record()
         mark("project", True)
-        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
+        # the projection writes the records, depth and boxes, and clears the
+        # adjoint's accumulators
+        gacc = torch.empty(9 * args.num_gauss, device=dev, dtype=torch.float32)
+        pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape, with_conic=False,
+                                              grad_acc=gacc)
         mark("project", False)
         mark("bin", True)
         t2i, i2p = bin_frame(pxd, aabb, i)
@@ -348,12 +352,7 @@ def headline(args, torch, np, SG, scene):
         order = SG.tile_order(cost)
         hint[0] = order
         mark("raster_fwd", False)
-        # the L2 loss gradient 2 (img - target) is formed inside the adjoint;
-        # this stage zeroes its accumulators, so the raster_bwd events bracket
-        # the adjoint kernel alone
-        mark("loss_grad", True)
-        gacc = torch.zeros(9 * splat2.shape[0], device=dev, dtype=torch.float32)
-        mark("loss_grad", False)
+        # the L2 loss gradient 2 (img - target) is formed inside the adjoint
         mark("raster_bwd", True)
         dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
                                                     img, target, args.threads_bwd,
@@ -599,7 +598,9 @@ def main():
     model = {
         "raster_fwd": 40 * K + 24 * P,
         "raster_bwd": 16 * P + 8 * P + 40 * K + 36 * N,
-        "project": 80 * N,
+        # reads mean, scale, quaternion, opacity, colour (56 B); writes
+        # pixel/depth, box, the 64-byte record and the cleared accumulators
+        "project": 56 * N + 28 * N + 64 * N + 36 * N,
         "project_bwd": 40 * N + 36 * N + 56 * N,
     }
     dom = max(("raster_fwd", "raster_bwd", "project", "project_bwd"), key=lambda k: stage_ms[k])

@@ -137,6 +137,14 @@ __device__ inline bool unit_depth_key(float ndcz, uint32_t i, unsigned long long
   return true;
 }
 
+// Keys only decrease, so a key that is not below the pixel's current one
+// (read without the atomic; a stale read is never below the true value)
+// skips the atomic: most do, since a pixel covered m times in random depth
+// order sees about ln m successful updates.
+__device__ inline void zbuffer_min(unsigned long long* p, unsigned long long key) {
+  if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, key);
+}
+
 __global__ void __launch_bounds__(256)
 point_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, uint32_t img_w, uint32_t img_h,
                      unsigned long long* __restrict__ keys) {
@@ -146,31 +154,95 @@ point_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, uint32_t img_w, 
   if (!unit_depth_key(pxz[3 * i + 2], i, &key)) return;
   const float fx = floorf(pxz[3 * i]), fy = floorf(pxz[3 * i + 1]);
   if (!(fx >= 0.f && fx < float(img_w) && fy >= 0.f && fy < float(img_h))) return;
-  atomicMin(&keys[size_t(fy) * img_w + size_t(fx)], key);
+  zbuffer_min(&keys[size_t(fy) * img_w + size_t(fx)], key);
 }
+
+// Per-wave LDS of circle_zbuffer_kernel: the 64 points' boxes and keys.
+struct ZbufLds {
+  uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there
+  uint32_t off[64], bw[64], klo[64], khi[64];
+  int x0[64], y0[64];
+  float x[64], y[64], r[64];
+};
 
 // Every pixel whose centre lies within the radius: centres ix + .5 in
 // [x - r, x + r], then the exact distance test of splat_circle.cu:120-125.
+// Each wave takes 64 points and deals the pixels of their clipped boxes out
+// 64 at a time (row-major within a box): the lanes stay busy whatever the
+// radii, and neighbouring lanes touch neighbouring pixels. A slot's point is
+// the last one whose range starts at or before it (each point marks its
+// first slot; a prefix max spreads the marks, carried across chunks).
 __global__ void __launch_bounds__(256)
 circle_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, const float* __restrict__ pixrad,
                       uint32_t img_w, uint32_t img_h, unsigned long long* __restrict__ keys) {
 #pragma clang fp contract(off)
+  __shared__ ZbufLds lds[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  unsigned long long key;
-  if (!unit_depth_key(pxz[3 * i + 2], i, &key)) return;
-  const float x = pxz[3 * i], y = pxz[3 * i + 1], r = pixrad[i];
-  if (!(r >= 0.f) || !(x == x) || !(y == y)) return;
-  const float fx0 = ceilf(x - r - 0.5f), fx1 = floorf(x + r - 0.5f);
-  const float fy0 = ceilf(y - r - 0.5f), fy1 = floorf(y + r - 0.5f);
-  const int x0 = int(fmaxf(fx0, 0.f)), x1 = int(fminf(fx1, float(img_w) - 1.f));
-  const int y0 = int(fmaxf(fy0, 0.f)), y1 = int(fminf(fy1, float(img_h) - 1.f));
-  for (int iy = y0; iy <= y1; ++iy)
-    for (int ix = x0; ix <= x1; ++ix) {
-      const float dx = x - (float(ix) + 0.5f), dy = y - (float(iy) + 0.5f);
-      if (!(sqrtf(dx * dx + dy * dy) <= r)) continue;
-      atomicMin(&keys[size_t(iy) * img_w + ix], key);
+  if ((i & ~63u) >= n) return;  // wave-uniform
+  const int lane = threadIdx.x & 63;
+  ZbufLds& L = lds[threadIdx.x >> 6];
+  unsigned long long key = 0;
+  uint32_t cnt = 0, bw = 0;
+  int x0 = 0, y0 = 0;
+  float x = 0.f, y = 0.f, r = 0.f;
+  if (i < n && unit_depth_key(pxz[3 * i + 2], i, &key)) {
+    x = pxz[3 * i];
+    y = pxz[3 * i + 1];
+    r = pixrad[i];
+    if (r >= 0.f && x == x && y == y) {
+      const float fx0 = ceilf(x - r - 0.5f), fx1 = floorf(x + r - 0.5f);
+      const float fy0 = ceilf(y - r - 0.5f), fy1 = floorf(y + r - 0.5f);
+      // clamped to the image in float before the conversion (a box off the
+      // image comes out empty)
+      x0 = int(fminf(fmaxf(fx0, 0.f), float(img_w)));
+      y0 = int(fminf(fmaxf(fy0, 0.f), float(img_h)));
+      const int x1 = int(fmaxf(fminf(fx1, float(img_w) - 1.f), -1.f));
+      const int y1 = int(fmaxf(fminf(fy1, float(img_h) - 1.f), -1.f));
+      if (x1 >= x0 && y1 >= y0) {
+        bw = uint32_t(x1 - x0 + 1);
+        cnt = bw * uint32_t(y1 - y0 + 1);
+      }
     }
+  }
+  const uint32_t inc = wave_incl_sum_u32(cnt);
+  const uint32_t off = inc - cnt;
+  const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
+  L.off[lane] = off;
+  L.bw[lane] = bw;
+  L.klo[lane] = uint32_t(key);
+  L.khi[lane] = uint32_t(key >> 32);
+  L.x0[lane] = x0;
+  L.y0[lane] = y0;
+  L.x[lane] = x;
+  L.y[lane] = y;
+  L.r[lane] = r;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+    L.own[lane] = 0;
+    wave_lds_fence();
+    if (cnt > 0 && off >= c0 && off - c0 < 64u) L.own[off - c0] = uint32_t(lane) + 1u;
+    wave_lds_fence();
+    const uint32_t own = wave_incl_max_u32(max(L.own[lane], lane == 0 ? carry : 0u));
+    carry = __builtin_amdgcn_readlane(own, 63);
+    const uint32_t e = c0 + uint32_t(lane);
+    if (e < total) {
+      const uint32_t lo = own - 1u;
+      const uint32_t c = e - L.off[lo];
+      const uint32_t w = L.bw[lo];
+      // c / w via the float reciprocal, corrected to the exact quotient
+      // (c < 2^22: off by at most one)
+      uint32_t q = uint32_t(float(c) * __builtin_amdgcn_rcpf(float(w)));
+      while (q * w > c) --q;
+      while ((q + 1u) * w <= c) ++q;
+      const int iy = L.y0[lo] + int(q), ix = L.x0[lo] + int(c - q * w);
+      const float dx = L.x[lo] - (float(ix) + 0.5f), dy = L.y[lo] - (float(iy) + 0.5f);
+      if (sqrtf(dx * dx + dy * dy) <= L.r[lo])
+        zbuffer_min(&keys[size_t(iy) * img_w + ix],
+                    (static_cast<unsigned long long>(L.khi[lo]) << 32) | L.klo[lo]);
+    }
+    // the next chunk rewrites own[] only after every lane has read it
+    wave_lds_fence();
+  }
 }
 
 __global__ void __launch_bounds__(256)

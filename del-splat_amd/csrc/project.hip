@@ -11,7 +11,7 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
                      const float* __restrict__ rgb, const float* __restrict__ m,
                      uint32_t img_w, uint32_t img_h, float* __restrict__ pixxydepth,
                      float* __restrict__ conic, float4* __restrict__ aabb,
-                     float4* __restrict__ splat2) {
+                     float4* __restrict__ splat2, float* __restrict__ zero9) {
   // the packed records leave through LDS: each wave writes its 64 records
   // (4 KB) with four fully contiguous 1 KB stores instead of four strided
   // quarter-record stores (which leave every line partly written by each
@@ -21,6 +21,16 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
   const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
   const uint32_t g0 = blockIdx.x * blockDim.x + wbase;  // first Gaussian of the wave
   if (g0 >= n) return;  // wave-uniform
+  if (zero9) {
+    // the adjoint's accumulators of the wave's Gaussians (9 floats each),
+    // cleared with coalesced stores
+    const uint32_t z1 = 9u * min(g0 + 64u, n);
+#pragma unroll
+    for (uint32_t k = 0; k < 9; ++k) {
+      const uint32_t z = 9u * g0 + k * 64u + lane;
+      if (z < z1) zero9[z] = 0.f;
+    }
+  }
   if (i < n) {
     const float4 q = quat[i];
     float s0 = scale[3 * i], s1 = scale[3 * i + 1], s2 = scale[3 * i + 2];
@@ -35,9 +45,11 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
     pixxydepth[3 * i + 0] = p.pix[0];
     pixxydepth[3 * i + 1] = p.pix[1];
     pixxydepth[3 * i + 2] = p.ndcz;
-    conic[3 * i + 0] = p.conic[0];
-    conic[3 * i + 1] = p.conic[1];
-    conic[3 * i + 2] = p.conic[2];
+    if (conic) {  // the packed record holds the conic too
+      conic[3 * i + 0] = p.conic[0];
+      conic[3 * i + 1] = p.conic[1];
+      conic[3 * i + 2] = p.conic[2];
+    }
     aabb[i] = make_float4(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3]);
     if (kPacked) {
       float4* mine = srec + 4 * threadIdx.x;
@@ -102,7 +114,8 @@ gauss_project_bwd_kernel(uint32_t n, const float* __restrict__ xyz,
 int launch_gauss_project(hipStream_t s, uint32_t n, const float* xyz, const float* scale,
                          const float* quat, const float* opacity, const float* rgb,
                          const float* m, uint32_t w, uint32_t h, bool log_scale,
-                         float* pixxydepth, float* conic, float* aabb, float* splat2) {
+                         float* pixxydepth, float* conic, float* aabb, float* splat2,
+                         float* zero9) {
   if (n == 0) return DSPLAT_OK;
   const uint32_t nb = div_up(n, 256);
   const float4* q4 = reinterpret_cast<const float4*>(quat);
@@ -110,14 +123,14 @@ int launch_gauss_project(hipStream_t s, uint32_t n, const float* xyz, const floa
   float4* rec = reinterpret_cast<float4*>(splat2);
   if (splat2) {
     if (log_scale)
-      gauss_project_kernel<true, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec);
+      gauss_project_kernel<true, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
     else
-      gauss_project_kernel<false, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec);
+      gauss_project_kernel<false, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
   } else {
     if (log_scale)
-      gauss_project_kernel<true, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec);
+      gauss_project_kernel<true, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
     else
-      gauss_project_kernel<false, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec);
+      gauss_project_kernel<false, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
   }
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
@@ -142,7 +155,7 @@ extern "C" int dsplat_gauss_pnt2splat3_to_pnt2splat2(void* stream, uint32_t num_
     return dsplat_set_error(DSPLAT_ERR_ARG, "gauss projection: quat/aabb must be 16-byte aligned");
   return launch_gauss_project(as_stream(stream), num_pnt, pnt2xyz, pnt2scale, pnt2quat, nullptr,
                               nullptr, transform_world2ndc, img_w, img_h, false, pnt2pixxydepth,
-                              pnt2pixconvinv, pnt2pixaabb, nullptr);
+                              pnt2pixconvinv, pnt2pixaabb, nullptr, nullptr);
 }
 
 extern "C" int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,
@@ -151,17 +164,16 @@ extern "C" int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const fl
                                         const float* transform_world2ndc, uint32_t img_w,
                                         uint32_t img_h, uint32_t flags, float* pnt2pixxydepth,
                                         float* pnt2pixconvinv, float* pnt2pixaabb,
-                                        float* pnt2splat2) {
+                                        float* pnt2splat2, float* d_grad9) {
   if (num_pnt && (!pnt2xyz || !pnt2scale || !pnt2quat || !pnt2opacity || !pnt2rgb ||
-                  !pnt2pixxydepth || !pnt2pixconvinv || !pnt2pixaabb || !pnt2splat2 ||
-                  !transform_world2ndc))
+                  !pnt2pixxydepth || !pnt2pixaabb || !pnt2splat2 || !transform_world2ndc))
     return dsplat_set_error(DSPLAT_ERR_ARG, "gauss project_fwd: null buffer");
   if (!aligned16(pnt2quat) || !aligned16(pnt2pixaabb) || !aligned16(pnt2splat2))
     return dsplat_set_error(DSPLAT_ERR_ARG, "gauss project_fwd: quat/aabb/splat2 must be 16-byte aligned");
   return launch_gauss_project(as_stream(stream), num_pnt, pnt2xyz, pnt2scale, pnt2quat,
                               pnt2opacity, pnt2rgb, transform_world2ndc, img_w, img_h,
                               (flags & DSPLAT_SCALE_IS_LOG) != 0, pnt2pixxydepth, pnt2pixconvinv,
-                              pnt2pixaabb, pnt2splat2);
+                              pnt2pixaabb, pnt2splat2, d_grad9);
 }
 
 extern "C" int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,

@@ -102,20 +102,32 @@ def rasterize_pnt2splat2(img_shape, pnt2pixxydepth, pnt2pixconvinv, pnt2pixaabb,
 
 
 # ------------------------------------------------------------------ render --
-def project_fwd(xyz, scale, quat, opacity, rgb, m16, img_shape, log_scale=True):
+def project_fwd(xyz, scale, quat, opacity, rgb, m16, img_shape, log_scale=True,
+                with_conic=True, grad_acc=None):
     """Projection with the packed raster record. Returns (pixxydepth, conic,
-    aabb, splat2 [N,16])."""
+    aabb, splat2 [N,16]); conic is None without with_conic (the records hold
+    it). grad_acc: a float32 [9N] tensor the kernel sets to zero (the
+    accumulators rasterize_bwd / rasterize_bwd_l2 take as grad_acc)."""
     dev = xyz.device
     n = xyz.shape[0]
     pxd = torch.empty((n, 3), device=dev, dtype=torch.float32)
-    conic = torch.empty((n, 3), device=dev, dtype=torch.float32)
+    conic = torch.empty((n, 3), device=dev, dtype=torch.float32) if with_conic else None
     aabb = torch.empty((n, 4), device=dev, dtype=torch.float32)
     splat2 = torch.empty((n, 16), device=dev, dtype=torch.float32)
+    if grad_acc is not None:
+        _check_grad_acc(grad_acc, n, dev)
     call("dsplat_gauss_project_fwd", stream_ptr(dev), n, ptr(xyz), ptr(scale), ptr(quat),
          ptr(opacity), ptr(rgb), ptr(m16), img_shape[0], img_shape[1],
          _native.DSPLAT_SCALE_IS_LOG if log_scale else 0, ptr(pxd), ptr(conic), ptr(aabb),
-         ptr(splat2))
+         ptr(splat2), ptr(grad_acc))
     return pxd, conic, aabb, splat2
+
+
+def _check_grad_acc(grad_acc, n, dev):
+    if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
+            or not grad_acc.is_contiguous() or grad_acc.device != dev):
+        raise ValueError("grad_acc must be a contiguous float32 [9N] tensor on the "
+                         "splats' device")
 
 
 def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0, tile_order=None,
@@ -169,10 +181,7 @@ def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast,
     if grad_acc is None:
         acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
     else:
-        if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
-                or not grad_acc.is_contiguous() or grad_acc.device != dev):
-            raise ValueError("grad_acc must be a contiguous float32 [9N] tensor on the "
-                             "splats' device")
+        _check_grad_acc(grad_acc, n, dev)
         acc = grad_acc.view(-1)
     d_pixxy = acc[:2 * n].view(n, 2)
     d_conic = acc[2 * n:5 * n].view(n, 3)
@@ -299,10 +308,7 @@ def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tla
     if grad_acc is None:
         acc = torch.zeros(9 * n, device=dev, dtype=torch.float32)
     else:
-        if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
-                or not grad_acc.is_contiguous() or grad_acc.device != dev):
-            raise ValueError("grad_acc must be a contiguous float32 [9N] tensor on the "
-                             "splats' device")
+        _check_grad_acc(grad_acc, n, dev)
         acc = grad_acc.view(-1)
     if tile_order is not None:
         _check_schedule(tile2idx, tile_order)
@@ -331,8 +337,10 @@ class _Render(torch.autograd.Function):
                 threads_bwd):
         xyz, log_scale, quat, opacity, rgb = (t.contiguous() for t in
                                               (xyz, log_scale, quat, opacity, rgb))
-        pxd, conic, aabb, splat2 = project_fwd(xyz, log_scale, quat, opacity, rgb, m16,
-                                               img_shape, log_scale=True)
+        # the projection also clears the adjoint's accumulators
+        gacc = torch.empty(9 * xyz.shape[0], device=xyz.device, dtype=torch.float32)
+        pxd, _, aabb, splat2 = project_fwd(xyz, log_scale, quat, opacity, rgb, m16, img_shape,
+                                           log_scale=True, with_conic=False, grad_acc=gacc)
         tile2idx, idx2pnt = bin_gauss(img_shape, tile_size, pxd, aabb)
         # schedule: the previous frame's order at this image / tile size as a
         # hint for the forward, this frame's walks for the adjoint
@@ -346,6 +354,7 @@ class _Render(torch.autograd.Function):
         _SCHEDULE[key] = order
         ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast,
                               order)
+        ctx.gacc = gacc  # zeroed; handed to the first backward only
         ctx.img_shape = img_shape
         ctx.tile_size = tile_size
         ctx.threads = threads_bwd
@@ -358,9 +367,11 @@ class _Render(torch.autograd.Function):
          order) = ctx.saved_tensors
         if d_img is None:
             d_img = torch.zeros((ctx.img_shape[1], ctx.img_shape[0], 3), device=xyz.device)
+        gacc, ctx.gacc = ctx.gacc, None
         d_pixxy, d_conic, d_op, d_rgb = rasterize_bwd(ctx.img_shape, ctx.tile_size, splat2,
                                                       tile2idx, idx2pnt, nwalk, tlast, d_img,
-                                                      d_alpha, ctx.threads, tile_order=order)
+                                                      d_alpha, ctx.threads, tile_order=order,
+                                                      grad_acc=gacc)
         d_xyz, d_ls, d_quat = project_bwd(xyz, log_scale, quat, m16, ctx.img_shape, d_pixxy,
                                           d_conic, log_scale=True)
         return d_xyz, d_ls, d_quat, d_op, d_rgb, None, None, None, None, None

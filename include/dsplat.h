@@ -163,13 +163,18 @@ int dsplat_gauss_rasterize_pnt2splat2(void* stream, uint32_t img_w, uint32_t img
 /* Projection for the differentiable path: as
  * dsplat_gauss_pnt2splat3_to_pnt2splat2 (flags may select log-scales) and in
  * addition the packed per-Gaussian raster record pnt2splat2 [N,16] float
- * (x, y, a, b, c, opacity, r, g, b, x0, y0, x1, y1, 0, 0, 0). */
+ * (x, y, a, b, c, opacity, r, g, b, x0, y0, x1, y1, 0, 0, 0).
+ * pnt2pixconvinv may be null (the conics are then only in the records; the
+ * raster kernels and the adjoints read nothing else). d_grad9 (may be null):
+ * float [9N] set to zero -- the accumulators the raster adjoint adds into,
+ * laid out d_pnt2pixxy [N,2] | d_pnt2conic [N,3] | d_pnt2opacity [N] |
+ * d_pnt2rgb [N,3] -- so that no separate clearing pass runs before it. */
 int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,
                              const float* pnt2scale, const float* pnt2quat,
                              const float* pnt2opacity, const float* pnt2rgb,
                              const float* transform_world2ndc, uint32_t img_w, uint32_t img_h,
                              uint32_t flags, float* pnt2pixxydepth, float* pnt2pixconvinv,
-                             float* pnt2pixaabb, float* pnt2splat2);
+                             float* pnt2pixaabb, float* pnt2splat2, float* d_grad9);
 
 /* Tiled raster over packed records, with the state the adjoint needs:
  * pix2rgb [H,W,3], pix2alpha [H,W] (the reference's alpha_sum), pix2nwalk

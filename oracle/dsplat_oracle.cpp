@@ -835,12 +835,16 @@ void zbuffer(uint32_t n, const float* pixxydepth, const float* pixrad, const flo
       continue;
     }
     const float r = pixrad[i];
-    if (!(r >= 0.f)) continue;
-    // pixel centres ix + 0.5 within [x - r, x + r]
+    if (!(r >= 0.f) || std::isnan(pxz[0]) || std::isnan(pxz[1])) continue;
+    // pixel centres ix + 0.5 within [x - r, x + r], clamped to the image in
+    // float before the conversion (a box off the image comes out empty)
     const float fx0 = std::ceil(pxz[0] - r - 0.5f), fx1 = std::floor(pxz[0] + r - 0.5f);
     const float fy0 = std::ceil(pxz[1] - r - 0.5f), fy1 = std::floor(pxz[1] + r - 0.5f);
-    const int x0 = int(std::max(fx0, 0.f)), x1 = int(std::min(fx1, float(img_w) - 1.f));
-    const int y0 = int(std::max(fy0, 0.f)), y1 = int(std::min(fy1, float(img_h) - 1.f));
+    // (fmax / fmin: a NaN bound gives way to the other operand)
+    const int x0 = int(std::fmin(std::fmax(fx0, 0.f), float(img_w)));
+    const int x1 = int(std::fmax(std::fmin(fx1, float(img_w) - 1.f), -1.f));
+    const int y0 = int(std::fmin(std::fmax(fy0, 0.f), float(img_h)));
+    const int y1 = int(std::fmax(std::fmin(fy1, float(img_h) - 1.f), -1.f));
     for (int iy = y0; iy <= y1; ++iy)
       for (int ix = x0; ix <= x1; ++ix) {
         const float dx = pxz[0] - (float(ix) + 0.5f), dy = pxz[1] - (float(iy) + 0.5f);

@@ -107,3 +107,22 @@ def check_pixels(g_img, g_alpha, g_nwalk, o_img, o_alpha, o_nwalk, tol=1e-4, fli
         print(f"flips {int(flip.sum())} vs the oracle's own at +-1 ulp of exp: {ns}", flush=True)
         assert flip.sum() <= ns, (int(flip.sum()), ns)
     return int(flip.sum())
+
+
+def zbuffer_edge_points(rng, shape, n):
+    """Random circles plus the edge rows: NaN centres, radius 0 on a pixel
+    centre, negative / infinite / huge radii, centres far off the image,
+    depths outside [-1, 1], equal depths."""
+    w, h = shape
+    pxz = np.stack([rng.uniform(-4, w + 4, n), rng.uniform(-4, h + 4, n),
+                    rng.uniform(-1.1, 1.1, n)], 1).astype(np.float32)
+    rad = rng.uniform(0.0, 4.0, n).astype(np.float32)
+    edge = [((np.nan, 3.0, 0.1), 2.0), ((3.0, np.nan, 0.1), 2.0), ((3.5, 4.5, -0.3), 0.0),
+            ((5.0, 5.0, 0.2), -1.0), ((7.2, 6.1, 0.9), np.inf), ((1e30, 5.0, 0.0), 3.0),
+            ((-1e30, 5.0, 0.0), 3.0), ((4.0, 1e30, 0.0), 1e30), ((6.0, 7.0, 0.95), 1e30),
+            ((np.inf, 5.0, 0.0), np.inf), ((2.0, 2.0, 1.5), 3.0), ((9.5, 3.5, 0.25), 2.5),
+            ((9.5, 3.5, 0.25), 2.5)]
+    for k, (c, r) in enumerate(edge):
+        pxz[k] = c
+        rad[k] = r
+    return pxz, rad

@@ -100,3 +100,32 @@ def test_point_zbuffer_and_painter(oracle):
     # and through CPU tensors (staged via the GPU)
     cimg = SP.rasterize_pix(torch.from_numpy(o_pxd), torch.from_numpy(p["rgb"]), shape)
     np.testing.assert_array_equal(cimg.numpy(), pimg)
+
+
+@pytest.mark.parametrize("with_radius", [True, False])
+def test_zbuffer_edge_cases(oracle, with_radius):
+    """Edge rows (NaN centres, radius 0 on a pixel centre, negative, infinite
+    and huge radii, centres far off the image, clipped depths, equal depths)
+    among random points: winners and colours bit-exact, for the circle and
+    the point z-buffer, at two sizes (the second not a multiple of 64)."""
+    import del_splat_dlpack.SplatCircle.torch as SC
+    import del_splat_dlpack.SplatPoint.torch as SP
+    from _helpers import zbuffer_edge_points
+    shape = (203, 117)
+    rng = np.random.default_rng(61)
+    pxz, rad = zbuffer_edge_points(rng, shape, 5000)
+    rgb = rng.random((pxz.shape[0], 3)).astype(np.float32)
+    for n in (5000, 1337):
+        print(f"zbuffer edge cases n={n} radius={with_radius}", flush=True)
+        if with_radius:
+            img, dep = SC.rasterize_zbuffer(_t(pxz[:n]), _t(rad[:n]), _t(rgb[:n]), shape)
+            keys, o_img = oracle.zbuffer(pxz[:n], rgb[:n], shape, pixrad=rad[:n])
+        else:
+            img, dep = SP.rasterize_pix_zbuffer(_t(pxz[:n]), _t(rgb[:n]), shape,
+                                                return_depth=True)
+            keys, o_img = oracle.zbuffer(pxz[:n], rgb[:n], shape)
+        np.testing.assert_array_equal(img.cpu().numpy(), o_img)
+        win = keys != np.iinfo(np.uint64).max
+        o_dep = np.where(win, (keys >> np.uint64(32)).astype(np.uint32).view(np.float32),
+                         np.float32(np.finfo(np.float32).max))
+        np.testing.assert_array_equal(dep.cpu().numpy(), o_dep)

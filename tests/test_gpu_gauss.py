@@ -369,6 +369,38 @@ def test_raster_bwd_fused_l2(ts, threads):
         assert max_rel(a.cpu().numpy(), b.cpu().numpy()) <= 1e-5
 
 
+@pytest.mark.parametrize("n", [20000, 1337])
+def test_project_fwd_clears_accumulators_record_only(n):
+    """project_fwd without the conic array (with_conic=False) writes the same
+    records, depths and boxes, and with grad_acc it sets the [9N]
+    accumulators to zero (filled with garbage first, n not a multiple of 64
+    included): the fused-L2 adjoint into them equals the adjoint into fresh
+    zeros up to the order of its float atomics (1e-6 of the largest)."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import scene
+    shape = (333, 201)
+    s, m = small_scene(n, 23, shape)
+    xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, _t(m), shape)
+    gacc = torch.full((9 * n + 64,), float("nan"), device=DEV)
+    pxd2, none, aabb2, splat22 = SG.project_fwd(xyz, ls, q, op, rgb, _t(m), shape,
+                                                with_conic=False, grad_acc=gacc[:9 * n])
+    assert none is None
+    for a, b in ((pxd, pxd2), (aabb, aabb2), (splat2, splat22)):
+        assert torch.equal(a, b)
+    np.testing.assert_array_equal(splat2.cpu().numpy()[:, 2:5], conic.cpu().numpy())
+    assert (gacc[:9 * n] == 0).all() and gacc[9 * n:].isnan().all()
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, pxd, aabb)
+    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, 16, splat2, t2i, i2p)
+    target = _t(scene.target_image(shape, 78))
+    ref = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tlast, img, target)
+    got = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tlast, img, target,
+                              grad_acc=gacc[:9 * n])
+    for a, b in zip(got, ref):
+        d = (a - b).abs().max().item()
+        assert d <= 1e-6 * max(b.abs().max().item(), 1e-30), d
+
+
 def test_project_bwd_add_and_overwrite():
     """dsplat_gauss_project_bwd adds into the gradient buffers by default and
     stores into them with DSPLAT_GRAD_OVERWRITE: add(into ones) == 1 + store."""

@@ -5,7 +5,7 @@ differences, and the reference's own (commented) known-answer test."""
 import numpy as np
 import pytest
 
-from _helpers import check_grad, small_scene
+from _helpers import check_grad, small_scene, zbuffer_edge_points
 from del_splat_dlpack import camera, scene
 
 
@@ -465,6 +465,31 @@ def test_zbuffer_brute_force(oracle):
         idx = (best & np.uint64(0xFFFFFFFF)).astype(np.int64)
         np.testing.assert_array_equal(img[win], rgb[idx[win]])
         assert np.all(img[~win] == 0)
+
+
+def test_zbuffer_edge_cases_brute_force(oracle):
+    """Circle z-buffer on the edge rows of zbuffer_edge_points against the
+    per-pixel brute force (float32 distance test, min key)."""
+    rng = np.random.default_rng(16)
+    shape = (20, 14)
+    pxz, rad = zbuffer_edge_points(rng, shape, 80)
+    rgb = rng.random((pxz.shape[0], 3)).astype(np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        keys, _ = oracle.zbuffer(pxz, rgb, shape, pixrad=rad)
+        best = np.full((shape[1], shape[0]), np.iinfo(np.uint64).max, np.uint64)
+        for i in range(pxz.shape[0]):
+            z = pxz[i, 2]
+            if not (-1 <= z <= 1) or not (rad[i] >= 0):
+                continue
+            u = np.float32(1.0 - (float(z) + 1.0) * 0.5)
+            key = (np.uint64(u.view(np.uint32)) << np.uint64(32)) | np.uint64(i)
+            ix, iy = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]))
+            dx = pxz[i, 0] - (ix.astype(np.float32) + np.float32(0.5))
+            dy = pxz[i, 1] - (iy.astype(np.float32) + np.float32(0.5))
+            hit = np.sqrt(dx * dx + dy * dy) <= rad[i]
+            best = np.where(hit & (key < best), key, best)
+    np.testing.assert_array_equal(keys, best)
+    assert (keys != np.iinfo(np.uint64).max).all()  # the infinite radius covers every pixel
 
 
 def test_point_painter_matches_python(oracle):
