@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--no-sections", action="store_true",
                     help="skip the forward (100k, 512x512) and z-buffer (2M points) sections")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--only", choices=("forward", "zbuffer", "batched", "train"), default=None,
+                    help="run only this section and print its JSON object (for profiling one "
+                         "workload in a process of its own)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--train-gauss", type=int, default=3_000_000)
@@ -591,6 +594,15 @@ def main():
     import del_splat_dlpack.SplatGauss.torch as SG
     from del_splat_dlpack import scene
 
+    if args.only:
+        if args.threads_bwd is None:
+            args.threads_bwd = args.threads
+        run = {"forward": lambda: section_forward(args, torch, np, SG, scene),
+               "zbuffer": lambda: section_zbuffer(args, torch, np, scene),
+               "batched": lambda: batched(args, torch, np, SG, scene),
+               "train": lambda: train(args, torch, np, SG, scene)}[args.only]
+        print(json.dumps({"section": args.only, **run()}), flush=True)
+        return
     s, m, shape, ms, stage_ms, K, evals = headline(args, torch, np, SG, scene)
     N = args.num_gauss
     P = shape[0] * shape[1]

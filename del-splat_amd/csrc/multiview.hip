@@ -618,6 +618,22 @@ extern "C" int dsplat_mv_plan_destroy(dsplat_mv_plan* P) {
   return DSPLAT_OK;
 }
 
+// The view table travels as kernel arguments (no host buffer has to outlive
+// the call, and the host never waits): 16 views per launch.
+constexpr uint32_t kViewsPerUpload = 16;
+struct ViewUpload {
+  ViewDev v[kViewsPerUpload];
+};
+static_assert(sizeof(ViewUpload) <= 2048, "view upload must fit the kernel arguments");
+
+__global__ void __launch_bounds__(256)
+upload_views_kernel(ViewUpload up, uint32_t count, ViewDev* __restrict__ dst) {
+  constexpr uint32_t kWords = sizeof(ViewDev) / 4;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(up.v);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  for (uint32_t k = threadIdx.x; k < count * kWords; k += blockDim.x) d[k] = src[k];
+}
+
 // A captured graph bakes in view counts, tile totals, grid sizes and the SH
 // mode; changing any of them drops it (the next dsplat_mv_graph_launch then
 // reports that nothing is captured instead of replaying stale launches).
@@ -634,11 +650,15 @@ extern "C" int dsplat_mv_set_views(dsplat_mv_plan* P, void* stream, uint32_t num
   drop_graph(P);
   P->fwd_valid = false;
   if (int rc = fill_views(P, num_view, views)) return rc;
-  DSPLAT_HIP(hipMemcpyAsync(P->d_views, P->host_views.data(), sizeof(ViewDev) * num_view,
-                            hipMemcpyHostToDevice, as_stream(stream)));
-  if (int rc = iota_u32(as_stream(stream), P->total_tiles, P->tile_order)) return rc;
-  DSPLAT_HIP(hipStreamSynchronize(as_stream(stream)));
-  return DSPLAT_OK;
+  hipStream_t s = as_stream(stream);
+  for (uint32_t v0 = 0; v0 < num_view; v0 += kViewsPerUpload) {
+    ViewUpload up{};
+    const uint32_t cnt = min(kViewsPerUpload, num_view - v0);
+    memcpy(up.v, P->host_views.data() + v0, sizeof(ViewDev) * cnt);
+    upload_views_kernel<<<1, 256, 0, s>>>(up, cnt, P->d_views + v0);
+  }
+  DSPLAT_HIP(hipGetLastError());
+  return iota_u32(s, P->total_tiles, P->tile_order);
 }
 
 extern "C" int dsplat_mv_set_sh(dsplat_mv_plan* P, uint32_t degree, uint32_t num_coef) {

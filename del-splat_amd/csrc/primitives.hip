@@ -121,8 +121,19 @@ int exclusive_scan(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, const uin
   return DSPLAT_OK;
 }
 
+// Up to this many chunk sums are scanned by one workgroup; more go through
+// the three-kernel scan (one workgroup walking ~10^5 sums takes ~100 us).
+constexpr uint32_t kSinglePassPartials = 16384;
+
+__global__ void copy_word_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  *dst = *src;
+}
+
 size_t scan_chunks_workspace_bytes(uint32_t n) {
-  return (size_t(div_up(n, kScanBlock)) + 2) * sizeof(uint32_t) + 256;
+  const uint32_t nb = div_up(n, kScanBlock);
+  size_t b = (size_t(nb) + 2) * sizeof(uint32_t) + 256;
+  if (nb > kSinglePassPartials) b = ((b + 255) & ~size_t(255)) + scan_workspace_bytes(nb);
+  return b;
 }
 
 int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t* ws,
@@ -134,7 +145,14 @@ int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t*
   }
   const uint32_t nb = div_up(n, kScanBlock);
   scan_reduce_kernel<1><<<nb, kScanBlock, 0, s>>>(n, nullptr, in, ws);
-  scan_partials_kernel<<<1, kScanBlock, 0, s>>>(nb, ws, d_total);
+  if (nb <= kSinglePassPartials) {
+    scan_partials_kernel<<<1, kScanBlock, 0, s>>>(nb, ws, d_total);
+  } else {
+    const size_t head = ((size_t(nb) + 2) * sizeof(uint32_t) + 256 + 255) & ~size_t(255);
+    uint32_t* ws2 = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + head);
+    if (int rc = exclusive_scan(s, nb, nullptr, ws, ws, ws2, ws + nb)) return rc;
+    if (d_total) copy_word_kernel<<<1, 1, 0, s>>>(ws + nb, d_total);
+  }
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }

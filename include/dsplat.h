@@ -386,9 +386,13 @@ int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64_t max_pixe
                           dsplat_mv_plan** plan);
 int dsplat_mv_plan_destroy(dsplat_mv_plan* plan);
 
-/* SYNC. Sets the views of the next frames (host array of num_view). Images
- * are concatenated in view order: view v's pixels start at pix_offsets[v].
- * Drops a captured graph (its launches were sized for the old views). */
+/* Sets the views of the next frames (host array of num_view, read before the
+ * call returns). Asynchronous: the view table is written on `stream` by
+ * kernels that take it as their arguments, so frames queued on `stream`
+ * before the call still see the old views and frames after it the new ones
+ * (give the plan's work one stream). Images are concatenated in view order:
+ * view v's pixels start at pix_offsets[v]. Drops a captured graph (its
+ * launches were sized for the old views). */
 int dsplat_mv_set_views(dsplat_mv_plan* plan, void* stream, uint32_t num_view,
                         const dsplat_view_desc* views);
 

@@ -158,6 +158,20 @@ def test_binning_large_tile_grid(oracle):
     np.testing.assert_array_equal(u32(i2p), o_i2p)
 
 
+def test_binning_many_gaussians(oracle):
+    """4.5M Gaussians: more than 16384 chunk sums, so the pair offsets take
+    the two-level scan; same lists as the oracle."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (320, 240)
+    s, m = small_scene(4_500_000, 9, shape)
+    pxd, conic, aabb = _project(s, m, shape)
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, pxd, aabb)
+    o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, 16)
+    print(f"pairs {len(o_i2p)}", flush=True)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+
+
 def _copy_device(dst, src, nbytes):
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
@@ -176,7 +190,7 @@ def test_capacity_overflow_reports_k(oracle):
     assert int(t2i.abs().sum().item()) == 0  # nothing binned on overflow
 
 
-@pytest.mark.parametrize("n", [1, 1000, 2048, 100_003])
+@pytest.mark.parametrize("n", [1, 1000, 2048, 4096, 4097, 100_003])
 @pytest.mark.parametrize("bits", [1, 13, 37, 64])
 def test_radix_sort_stable(n, bits):
     from del_splat_dlpack import _native

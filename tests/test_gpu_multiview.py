@@ -115,6 +115,32 @@ def test_graph_replay_forward():
     plan.close()
 
 
+def test_set_views_is_stream_ordered():
+    """set_views does not wait for the GPU: frames queued before it render the
+    old views, frames after it the new ones, with no synchronisation in
+    between (a slow frame first, so the table upload really overlaps it)."""
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
+    s, views = _setup(n=30000, seed=47)
+    n = s["xyz"].shape[0]
+    args = [_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb")]
+    other = scene.sphere_cameras(3, 48, resolutions=SHAPES)
+    refs = []
+    for vs in (views, other):
+        p = MultiViewPlan(n, vs, tile_size=16)
+        refs.append(p.forward(*args)[0].cpu().numpy())
+        p.close()
+    plan = MultiViewPlan(n, views, tile_size=16)
+    torch.cuda.synchronize()
+    outs = []
+    for vs in (views, other, views, other):
+        plan.set_views(vs)
+        outs.append(plan.forward(*args)[0])
+    torch.cuda.synchronize()
+    for k, o in enumerate(outs):
+        np.testing.assert_array_equal(o.cpu().numpy(), refs[k % 2])
+    plan.close()
+
+
 def test_changing_views_or_colour_drops_the_graph():
     """A captured graph is sized for its views; after set_views or set_sh a
     replay raises instead of running stale launches, and a new capture
