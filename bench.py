@@ -26,6 +26,9 @@ Z-buffer section: 2M points uniform in the unit cube, per-point pixel radius
 U[0.5,3], 1920x1080: point projection and the circle z-buffer (packed
 depth/index atomicMin per covered pixel, then the colour gather).
 
+SH colour section: degree-3 spherical-harmonic colour of 1M Gaussians, forward
+and adjoint.
+
 CPU baseline: the oracle (a C++ restatement of del-splat-cpu) per stage,
 median of 5 runs, single-threaded (as the reference's CPU code runs) and with
 OpenMP over the host cores.
@@ -65,9 +68,10 @@ def parse():
                          "semantics) instead of the no-sync capacity path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sections", action="store_true",
-                    help="skip the forward (100k, 512x512) and z-buffer (2M points) sections")
+                    help="skip the forward (100k, 512x512), z-buffer (2M points) and SH colour "
+                         "(1M, degree 3) sections")
     ap.add_argument("--cpu-reps", type=int, default=5)
-    ap.add_argument("--only", choices=("forward", "zbuffer", "batched", "train"), default=None,
+    ap.add_argument("--only", choices=("forward", "zbuffer", "sh", "batched", "train"), default=None,
                     help="run only this section and print its JSON object (for profiling one "
                          "workload in a process of its own)")
     ap.add_argument("--no-batched", action="store_true")
@@ -215,6 +219,56 @@ def section_forward(args, torch, np, SG, scene):
             "roofline": {"bound": "hbm", "kernel": "raster_fwd", "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "kernel_ms": round(fwd_ms, 4), "bytes": model}}
+
+
+def section_sh(args, torch, np, SG, scene):
+    """View-dependent colour of 1M Gaussians from degree-3 spherical
+    harmonics (16 coefficients x 3 channels), forward and adjoint (stored
+    gradients), one camera; each kernel bracketed by HIP events."""
+    dev = torch.device("cuda:0")
+    n = 1_000_000
+    rng = np.random.default_rng(6)
+    xyz = torch.from_numpy(scene.gaussians(n, 5)["xyz"]).to(dev)
+    sh = torch.from_numpy((0.3 * rng.standard_normal((n, 16, 3))).astype(np.float32)).to(dev)
+    cam = (0.0, 0.0, 3.0)
+    d_rgb = torch.from_numpy(rng.standard_normal((n, 3)).astype(np.float32)).to(dev)
+    d_sh, d_xyz = torch.empty_like(sh), torch.empty_like(xyz)
+    steps = max(args.steps, 20)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        SG.sh_to_rgb_fwd(xyz, sh, cam, 3)
+        if ev:
+            ev[1].record()
+        SG.sh_to_rgb_bwd(xyz, sh, cam, 3, d_rgb)
+        if ev:
+            ev[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(evs[i])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    fwd_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
+    bwd_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
+    # forward: xyz 12 + coefficients 192 in, rgb 12 out; adjoint: xyz,
+    # coefficients, d_rgb in, d_coefficients and d_xyz out
+    fb, bb = (12 + 192 + 12) * n, (12 + 192 + 12 + 192 + 12) * n
+    return {"workload": f"SH colour degree 3, {n} Gaussians, forward + adjoint",
+            "value": round(ms, 4), "unit": "ms/step", "steps": steps,
+            "forward": {"kernel_ms": round(fwd_ms, 4), "bytes": fb,
+                        "achieved": round(fb / (fwd_ms * 1e-3) / 1e9, 1),
+                        "frac": round(fb / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "roofline": {"bound": "hbm", "kernel": "sh adjoint",
+                         "achieved": round(bb / (bwd_ms * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(bb / (bwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "kernel_ms": round(bwd_ms, 4), "bytes": bb}}
 
 
 def section_zbuffer(args, torch, np, scene):
@@ -599,6 +653,7 @@ def main():
             args.threads_bwd = args.threads
         run = {"forward": lambda: section_forward(args, torch, np, SG, scene),
                "zbuffer": lambda: section_zbuffer(args, torch, np, scene),
+               "sh": lambda: section_sh(args, torch, np, SG, scene),
                "batched": lambda: batched(args, torch, np, SG, scene),
                "train": lambda: train(args, torch, np, SG, scene)}[args.only]
         print(json.dumps({"section": args.only, **run()}), flush=True)
@@ -659,6 +714,7 @@ def main():
     if not args.no_sections:
         out["forward_100k_512"] = section_forward(args, torch, np, SG, scene)
         out["zbuffer_2m_1080p"] = section_zbuffer(args, torch, np, scene)
+        out["sh_colour_1m_deg3"] = section_sh(args, torch, np, SG, scene)
     if not args.no_batched:
         out["batched"] = batched(args, torch, np, SG, scene)
     if not args.no_train:

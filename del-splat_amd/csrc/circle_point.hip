@@ -138,11 +138,12 @@ __device__ inline bool unit_depth_key(float ndcz, uint32_t i, unsigned long long
 }
 
 // Keys only decrease, so a key that is not below the pixel's current one
-// (read without the atomic; a stale read is never below the true value)
 // skips the atomic: most do, since a pixel covered m times in random depth
-// order sees about ln m successful updates.
+// order sees about ln m successful updates. The read is a plain cached load:
+// a stale value is never below the true one (it only lets a losing key
+// through to the atomic, which then changes nothing).
 __device__ inline void zbuffer_min(unsigned long long* p, unsigned long long key) {
-  if (key < __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(p, key);
+  if (key < *p) atomicMin(p, key);
 }
 
 __global__ void __launch_bounds__(256)
@@ -157,16 +158,36 @@ point_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, uint32_t img_w, 
   zbuffer_min(&keys[size_t(fy) * img_w + size_t(fx)], key);
 }
 
-// Per-wave LDS of circle_zbuffer_kernel: the 64 points' boxes and keys.
+// Per-wave LDS of circle_zbuffer_kernel: the 64 points' boxes, distance
+// bounds and keys, three 16-byte records each.
 struct ZbufLds {
   uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there
-  uint32_t off[64], bw[64], klo[64], khi[64];
-  int x0[64], y0[64];
-  float x[64], y[64], r[64];
+  uint4 box[64];     // first slot, box width, x0, y0
+  float4 geo[64];    // x, y, d2 bounds (inside below lo, outside from hi)
+  uint4 key[64];     // key lo, key hi, bits of r
 };
 
+// The distance test sqrtf(d2) <= r of splat_circle.cu:120-125 without the
+// square root where d2 is clearly on one side: with rr = r*r rounded, d2 <=
+// rr (1 - 2^-20) means d2 < r^2, so the rounded root is <= r; d2 >= rr (1 +
+// 2^-20) means the root exceeds r by more than half an ulp of r, so the
+// rounded root is > r. Between the two (and for infinite or tiny bounds)
+// the exact test decides.
+__device__ inline float2 circle_bounds(float r) {
+#pragma clang fp contract(off)
+  const float rr = r * r;
+  if (!(rr >= 1e-30f) || rr == __builtin_inff()) return make_float2(-1.f, __builtin_inff());
+  return make_float2(rr * 0.99999904632568359375f, rr * 1.00000095367431640625f);
+}
+
+__device__ inline bool circle_covers(float d2, float lo, float hi, float r) {
+  if (d2 <= lo) return true;
+  if (d2 >= hi && hi != __builtin_inff()) return false;
+  return sqrtf(d2) <= r;
+}
+
 // Every pixel whose centre lies within the radius: centres ix + .5 in
-// [x - r, x + r], then the exact distance test of splat_circle.cu:120-125.
+// [x - r, x + r], then the exact distance test (circle_covers).
 // Each wave takes 64 points and deals the pixels of their clipped boxes out
 // 64 at a time (row-major within a box): the lanes stay busy whatever the
 // radii, and neighbouring lanes touch neighbouring pixels. A slot's point is
@@ -207,15 +228,10 @@ circle_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, const float* __
   const uint32_t inc = wave_incl_sum_u32(cnt);
   const uint32_t off = inc - cnt;
   const uint32_t total = __builtin_amdgcn_readlane(inc, 63);
-  L.off[lane] = off;
-  L.bw[lane] = bw;
-  L.klo[lane] = uint32_t(key);
-  L.khi[lane] = uint32_t(key >> 32);
-  L.x0[lane] = x0;
-  L.y0[lane] = y0;
-  L.x[lane] = x;
-  L.y[lane] = y;
-  L.r[lane] = r;
+  const float2 bnd = circle_bounds(r);
+  L.box[lane] = make_uint4(off, bw, uint32_t(x0), uint32_t(y0));
+  L.geo[lane] = make_float4(x, y, bnd.x, bnd.y);
+  L.key[lane] = make_uint4(uint32_t(key), uint32_t(key >> 32), __float_as_uint(r), 0u);
   uint32_t carry = 0;
   for (uint32_t c0 = 0; c0 < total; c0 += 64) {
     L.own[lane] = 0;
@@ -227,18 +243,21 @@ circle_zbuffer_kernel(uint32_t n, const float* __restrict__ pxz, const float* __
     const uint32_t e = c0 + uint32_t(lane);
     if (e < total) {
       const uint32_t lo = own - 1u;
-      const uint32_t c = e - L.off[lo];
-      const uint32_t w = L.bw[lo];
+      const uint4 bx = L.box[lo];
+      const float4 g = L.geo[lo];
+      const uint32_t c = e - bx.x, w = bx.y;
       // c / w via the float reciprocal, corrected to the exact quotient
       // (c < 2^22: off by at most one)
       uint32_t q = uint32_t(float(c) * __builtin_amdgcn_rcpf(float(w)));
       while (q * w > c) --q;
       while ((q + 1u) * w <= c) ++q;
-      const int iy = L.y0[lo] + int(q), ix = L.x0[lo] + int(c - q * w);
-      const float dx = L.x[lo] - (float(ix) + 0.5f), dy = L.y[lo] - (float(iy) + 0.5f);
-      if (sqrtf(dx * dx + dy * dy) <= L.r[lo])
+      const int iy = int(bx.w) + int(q), ix = int(bx.z) + int(c - q * w);
+      const float dx = g.x - (float(ix) + 0.5f), dy = g.y - (float(iy) + 0.5f);
+      const float d2 = dx * dx + dy * dy;
+      const uint4 k = L.key[lo];
+      if (circle_covers(d2, g.z, g.w, __uint_as_float(k.z)))
         zbuffer_min(&keys[size_t(iy) * img_w + ix],
-                    (static_cast<unsigned long long>(L.khi[lo]) << 32) | L.klo[lo]);
+                    (static_cast<unsigned long long>(k.y) << 32) | k.x);
     }
     // the next chunk rewrites own[] only after every lane has read it
     wave_lds_fence();
