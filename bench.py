@@ -172,8 +172,9 @@ def cpu_baseline(s, m, shape, ts, reps):
 def section_forward(args, torch, np, SG, scene):
     """The plain forward of 100k Gaussians at 512x512 with 16x16 tiles
     (projection, binning without a host wait, forward raster with RGB and
-    alpha), one frame per step; the raster kernel bracketed by HIP events on
-    the library's stream (torch's current stream)."""
+    alpha), one frame per step, at the forward workgroup size that is fastest
+    on the frame; the raster kernel bracketed by HIP events on the library's
+    stream (torch's current stream)."""
     dev = torch.device("cuda:0")
     shape, ts, n = (512, 512), 16, 100_000
     s = scene.gaussians(n, 0)
@@ -181,9 +182,24 @@ def section_forward(args, torch, np, SG, scene):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
     xyz, ls, q, op, rgb = (t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
     m16 = t(m)
-    pxd0, _, aabb0, _ = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
-    k0 = SG.bin_gauss(shape, ts, pxd0, aabb0)[1].numel()
+    pxd0, _, aabb0, splat0 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
+    t2i0, i2p0 = SG.bin_gauss(shape, ts, pxd0, aabb0)
+    k0 = i2p0.numel()
     cap = k0 + k0 // 8 + 65536
+    # the forward's workgroup size, measured on this frame (the tile size is
+    # the workload's)
+    tune = {}
+    for th_c in (64, 128, 256):
+        for _ in range(2):
+            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c)
+        e1.record()
+        torch.cuda.synchronize()
+        tune[th_c] = e0.elapsed_time(e1) / 5
+    th = min(tune, key=tune.get)
     steps = max(args.steps, 20)
     d_k = [torch.zeros(1, device=dev, dtype=torch.int32) for _ in range(steps)]
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -194,7 +210,7 @@ def section_forward(args, torch, np, SG, scene):
         t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, d_k[i])
         if ev:
             ev[0].record()
-        out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
+        out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th)
         if ev:
             ev[1].record()
         return out
@@ -216,6 +232,7 @@ def section_forward(args, torch, np, SG, scene):
     ach = model / (fwd_ms * 1e-3) / 1e9
     return {"workload": f"3dgs forward, {n} Gaussians, {shape[0]}x{shape[1]}, tile {ts}",
             "value": round(ms, 4), "unit": "ms/frame", "steps": steps, "pairs_K": K,
+            "threads": th, "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
             "roofline": {"bound": "hbm", "kernel": "raster_fwd", "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "kernel_ms": round(fwd_ms, 4), "bytes": model}}

@@ -50,8 +50,11 @@ __device__ inline uint32_t depth_key(float ndcz) {
   return __float_as_uint(zp1);
 }
 
-// Half-open tile rectangle of a pixel box, clipped to the tile grid
-// (tile_acceleration.h:27-41). An inverted or NaN box covers no tile.
+// Half-open tile rectangle of a pixel box, clipped to the tile grid: tiles
+// floor(x0 / ts) .. floor(x1 / ts) along each axis, as the loops of
+// tile_acceleration.h:27-44 run them (a finite inverted box inside one tile
+// row or column covers that tile; one spanning a boundary covers none). A
+// NaN coordinate covers no tile.
 struct TileRect {
   int x0, x1, y0, y1;
   __device__ int count() const { return (x1 - x0) * (y1 - y0); }
@@ -67,7 +70,7 @@ __device__ inline int tile_floor(float v, float tsf, uint32_t lim) {
 __device__ inline TileRect tile_rect(float x0, float y0, float x1, float y1, uint32_t ts,
                                      uint32_t tw, uint32_t th) {
   TileRect r{0, 0, 0, 0};
-  if (!(x0 <= x1) || !(y0 <= y1)) return r;
+  if (x0 != x0 || y0 != y0 || x1 != x1 || y1 != y1) return r;
   const float tsf = float(ts);
   r.x0 = max(tile_floor(x0, tsf, tw), 0);
   r.x1 = min(tile_floor(x1, tsf, tw) + 1, int(tw));

@@ -237,10 +237,13 @@ uint32_t gauss_project(uint32_t n, const T* xyz, const T* quat, const T* scale,
 // ---------------------------------------------------------------------------
 struct TileRect { int x0, x1, y0, y1; };  // half-open, clipped to the grid
 
-// Tile rectangle of a box; an inverted or NaN box covers no tile.
+// Tile rectangle of a box: tiles floor(x0 / ts) .. floor(x1 / ts) along each
+// axis, as the loops of tile_acceleration.h:27-44 run them (so a finite
+// inverted box inside one tile row or column still covers that tile, and
+// one spanning a tile boundary none); a NaN coordinate covers no tile.
 inline TileRect tile_rect(const float* bb, uint32_t ts, uint32_t tw, uint32_t th) {
   TileRect r{0, 0, 0, 0};
-  if (!(bb[0] <= bb[2]) || !(bb[1] <= bb[3])) return r;
+  if (std::isnan(bb[0]) || std::isnan(bb[1]) || std::isnan(bb[2]) || std::isnan(bb[3])) return r;
   const float tsf = float(ts);
   // floor(x / ts) clamped to [-1, tw] before the integer conversion.
   auto fl = [&](float v, uint32_t lim) {

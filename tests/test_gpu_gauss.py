@@ -61,13 +61,16 @@ def test_binning_bit_exact(oracle, ts, n, shape, seed):
 
 def test_binning_edge_boxes(oracle):
     """Boxes off-screen, straddling every border, covering the whole image,
-    inverted (culled), NaN and exactly on tile boundaries."""
+    culled (inverted to infinity), finite inverted inside one tile (binned
+    to it, as the reference's loops do) and across tiles (none), NaN and
+    exactly on tile boundaries."""
     import del_splat_dlpack.SplatGauss.torch as SG
     shape, ts = (100, 70), 16
     boxes = np.array([
         [-50, -50, -10, -10], [110, 10, 200, 20], [-5, -5, 5, 5], [95, 65, 130, 90],
         [-1e9, -1e9, 1e9, 1e9], [np.inf, np.inf, -np.inf, -np.inf], [np.nan, 0, 10, 10],
         [16, 16, 32, 32], [15.999, 0, 16.0, 0.0], [0, 0, 0, 0], [99.9, 69.9, 99.95, 69.95],
+        [40, 30, 35, 20], [40, 30, 20, 20], [30, 60, 34, 50],
     ], np.float32)
     n = len(boxes)
     pxd = np.zeros((n, 3), np.float32)

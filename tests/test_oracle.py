@@ -188,14 +188,38 @@ def test_binning_brute_force(oracle, ts):
 def test_tile_rect_edges(oracle):
     boxes = np.array([[15.999, 0, 16.0, 0.0], [16, 16, 32, 32], [-5, -5, -1, -1],
                       [np.inf, np.inf, -np.inf, -np.inf], [np.nan, 0, 1, 1],
-                      [-1e30, -1e30, 1e30, 1e30]], np.float32)
+                      [-1e30, -1e30, 1e30, 1e30], [9, 5, 3, 2], [20, 5, 10, 8]], np.float32)
     r = oracle.tile_rects(boxes, 16, 4, 3)
     np.testing.assert_array_equal(r[0], [0, 2, 0, 1])   # x1 = 16 touches tile 1
     np.testing.assert_array_equal(r[1], [1, 3, 1, 3])
     assert (r[2, 1] - r[2, 0]) * (r[2, 3] - r[2, 2]) == 0
-    assert (r[3, 1] - r[3, 0]) * (r[3, 3] - r[3, 2]) == 0
+    assert (r[3, 1] - r[3, 0]) * (r[3, 3] - r[3, 2]) == 0  # culled: inverted to infinity
     assert (r[4, 1] - r[4, 0]) * (r[4, 3] - r[4, 2]) == 0
     np.testing.assert_array_equal(r[5], [0, 4, 0, 3])
+    # inverted inside one tile: the reference's floor loops still visit it
+    np.testing.assert_array_equal(r[6], [0, 1, 0, 1])
+    assert (r[7, 1] - r[7, 0]) * (r[7, 3] - r[7, 2]) == 0  # inverted across a tile boundary
+
+
+def test_tile_rect_matches_reference_loops(oracle):
+    """Against the loops of tile_acceleration.h:27-44 written out (ix from
+    floor(x0 / ts) up to floor(x1 / ts), tiles outside the grid skipped), on
+    random boxes of every orientation, partly off the grid."""
+    rng = np.random.default_rng(3)
+    ts, tw, th = 16, 7, 5
+    n = 4000
+    c = rng.uniform(-40, 150, (n, 2))
+    hw = rng.uniform(-30, 60, (n, 2))  # negative half-widths: inverted boxes
+    boxes = np.concatenate([c - hw, c + hw], 1).astype(np.float32)
+    boxes[::7, 2] = boxes[::7, 0] - rng.uniform(0, 3, len(boxes[::7]))  # inverted in x only
+    r = oracle.tile_rects(boxes, ts, tw, th)
+    for i, b in enumerate(boxes):
+        ix0, iy0 = int(np.floor(b[0] / np.float32(ts))), int(np.floor(b[1] / np.float32(ts)))
+        ix1, iy1 = int(np.floor(b[2] / np.float32(ts))) + 1, int(np.floor(b[3] / np.float32(ts))) + 1
+        want = {(ix, iy) for ix in range(ix0, ix1) if 0 <= ix < tw
+                for iy in range(iy0, iy1) if 0 <= iy < th}
+        got = {(ix, iy) for ix in range(r[i, 0], r[i, 1]) for iy in range(r[i, 2], r[i, 3])}
+        assert got == want, (i, b, r[i])
 
 
 # ----------------------------------------------------------------- raster ---
