@@ -172,9 +172,20 @@ struct SortLayout {
   void* kalt;
   uint32_t* valt;
   uint32_t* tot;   // [256] digit totals of the current pass
-  uint32_t* cnt;   // [256][tiles] per-tile digit counts, then their prefixes
+  uint32_t* cnt;   // [256][slots] per-tile digit counts, then their prefixes
   size_t total;
 };
+
+// Column slot of sort tile b among nb: workgroups are dispatched to the 8
+// XCDs round-robin, so the tiles of one XCD (b mod 8 alike) are made
+// adjacent. A column's words written by the upsweep then share cache lines
+// only with words written on the same XCD and merge in its L2; in tile
+// order every line took one word from each of the eight XCDs.
+constexpr uint32_t kXcds = 8;
+__host__ __device__ inline uint32_t col_stride(uint32_t nb) { return kXcds * div_up(nb, kXcds); }
+__device__ inline uint32_t col_slot(uint32_t b, uint32_t nb) {
+  return (b % kXcds) * div_up(nb, kXcds) + b / kXcds;
+}
 
 template <typename KeyT>
 SortLayout sort_layout(void* base, uint32_t n) {
@@ -190,7 +201,7 @@ SortLayout sort_layout(void* base, uint32_t n) {
   L.kalt = take(size_t(n) * sizeof(KeyT));
   L.valt = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
   L.tot = reinterpret_cast<uint32_t*>(take(size_t(256) * 4));
-  L.cnt = reinterpret_cast<uint32_t*>(take(size_t(256) * nb * 4));
+  L.cnt = reinterpret_cast<uint32_t*>(take(size_t(256) * col_stride(nb) * 4));
   L.total = o;
   return L;
 }
@@ -201,10 +212,13 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
                      uint32_t shift, uint32_t dbits, uint32_t nb, uint32_t* __restrict__ cnt) {
   __shared__ uint32_t h[256];
   const uint32_t n = load_n(n_cap, d_n);
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+  // tiles past n (the grid covers the capacity) have nothing to count: the
+  // column scan and the scatter stop at the last tile holding an element
+  if (base >= n) return;  // uniform
   const int lane = threadIdx.x & 63;
   h[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
   const uint32_t mask = (1u << dbits) - 1u;
   // all loads first, then the histogram
   KeyT kk[kSortItems];
@@ -230,7 +244,8 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     }
   }
   __syncthreads();
-  if (threadIdx.x <= mask) cnt[size_t(threadIdx.x) * nb + blockIdx.x] = h[threadIdx.x];
+  if (threadIdx.x <= mask)
+    cnt[size_t(threadIdx.x) * col_stride(nb) + col_slot(blockIdx.x, nb)] = h[threadIdx.x];
 }
 
 // Workgroup d: cnt[d][b] <- elements of digit d in tiles before b;
@@ -242,20 +257,20 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
   const uint32_t nbv = div_up(n, kSortTile);
   const uint32_t d = blockIdx.x;
   uint32_t carry = 0;
-  uint32_t* col = cnt + size_t(d) * nb;
+  uint32_t* col = cnt + size_t(d) * col_stride(nb);
   for (uint32_t b0 = 0; b0 < nbv; b0 += kSortTile) {
     const uint32_t i0 = b0 + threadIdx.x * kSortItems;
     uint32_t v[kSortItems], s = 0;
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-      v[k] = i0 + k < nbv ? col[i0 + k] : 0u;
+      v[k] = i0 + k < nbv ? col[col_slot(i0 + k, nb)] : 0u;
       s += v[k];
     }
     uint32_t chunk;
     uint32_t run = carry + block_exclusive_scan(s, &chunk);
 #pragma unroll
     for (int k = 0; k < kSortItems; ++k) {
-      if (i0 + k < nbv) col[i0 + k] = run;
+      if (i0 + k < nbv) col[col_slot(i0 + k, nb)] = run;
       run += v[k];
     }
     carry += chunk;
@@ -289,7 +304,8 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   // fetched now so their latency hides behind the ranking
   const bool live_digit = threadIdx.x < nbins;
   const uint32_t my_tot = live_digit ? tot[threadIdx.x] : 0u;
-  const uint32_t my_off = live_digit ? off[size_t(threadIdx.x) * nb + blockIdx.x] : 0u;
+  const uint32_t my_off =
+      live_digit ? off[size_t(threadIdx.x) * col_stride(nb) + col_slot(blockIdx.x, nb)] : 0u;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
