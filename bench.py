@@ -191,11 +191,11 @@ def section_forward(args, torch, np, SG, scene):
     tune = {}
     for th_c in (64, 128, 256):
         for _ in range(2):
-            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c)
+            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c, state=False)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(5):
-            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c)
+            SG.rasterize_fwd(shape, ts, splat0, t2i0, i2p0, th_c, state=False)
         e1.record()
         torch.cuda.synchronize()
         tune[th_c] = e0.elapsed_time(e1) / 5
@@ -210,7 +210,7 @@ def section_forward(args, torch, np, SG, scene):
         t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, d_k[i])
         if ev:
             ev[0].record()
-        out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th)
+        out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, state=False)
         if ev:
             ev[1].record()
         return out
@@ -220,7 +220,7 @@ def section_forward(args, torch, np, SG, scene):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        img, alpha, nwalk, _ = step(i, evs[i])
+        img, alpha, _, _ = step(i, evs[i])
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     ks = [int(k.item()) for k in d_k]

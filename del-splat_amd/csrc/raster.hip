@@ -8,7 +8,7 @@ namespace dsplat {
 // tile_order (may be null: raster order) maps workgroups to tiles, so the
 // longest walks start first; tile_cost (may be null) receives each tile's
 // longest walk.
-template <int TS, int NT, typename Loader>
+template <int TS, int NT, typename Loader, bool kState>
 __global__ void __launch_bounds__(NT)
 gauss_raster_fwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw, Loader ld,
                         const uint32_t* __restrict__ tile2idx,
@@ -18,7 +18,7 @@ gauss_raster_fwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw, Loader ld,
                         uint32_t* __restrict__ tile_cost) {
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   if (tile >= gridDim.x) return;  // uniform: an order entry outside the grid is skipped
-  raster_tile_fwd<TS, NT, Loader>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile],
+  raster_tile_fwd<TS, NT, Loader, kState>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile],
                                   tile2idx[tile + 1], ld, idx2pnt, pix2rgb, pix2alpha, pix2nwalk,
                                   pix2tlast, tile_cost, tile);
 }
@@ -52,10 +52,17 @@ int launch_fwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
   tile_grid(w, h, ts, &tw, &th);
   const uint32_t nt = tw * th;
   if (nt == 0) return DSPLAT_OK;
-#define DSPLAT_FWD(TS_, NT_)                                                                   \
-  gauss_raster_fwd_kernel<TS_, NT_, Loader><<<nt, NT_, 0, s>>>(w, h, tw, ld, tile2idx, idx2pnt, \
-                                                               rgb, alpha, nwalk, tlast, order, \
-                                                               cost)
+  // the adjoint's state and the tile costs only when asked for
+  const bool state = nwalk || tlast || cost;
+#define DSPLAT_FWD(TS_, NT_)                                                                     \
+  do {                                                                                           \
+    if (state)                                                                                   \
+      gauss_raster_fwd_kernel<TS_, NT_, Loader, true><<<nt, NT_, 0, s>>>(                        \
+          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tlast, order, cost);                \
+    else                                                                                         \
+      gauss_raster_fwd_kernel<TS_, NT_, Loader, false><<<nt, NT_, 0, s>>>(                       \
+          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tlast, order, cost);                \
+  } while (0)
   if (ts == 8) {
     DSPLAT_FWD(8, 64);
   } else if (ts == 16) {

@@ -281,7 +281,9 @@ __device__ __forceinline__ bool box_take(int g, const bool (&walking)[GroupMap<T
 // Forward of one tile. Image arrays are the view's base pointers (pixel
 // index iy * img_w + ix); alpha / nwalk / tlast may be null. The entry loop is
 // outermost so one LDS read of a record serves all PPT pixels of a thread.
-template <int TS, int NT, typename Loader>
+// kState: track the adjoint's state (nwalk, tlast) and the tile cost; a
+// forward asked for neither leaves their two selects per pixel and entry out.
+template <int TS, int NT, typename Loader, bool kState = true>
 __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32_t img_w,
                                                 uint32_t img_h, uint32_t start, uint32_t end,
                                                 const Loader& ld,
@@ -377,8 +379,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cr[k] = fmaf(q1.z, eo, cr[k]);
                 cg[k] = fmaf(q1.w, eo, cg[k]);
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
-                tl[k] = in ? tr[k] : tl[k];
-                walked[k] = in ? b + j + 1 : walked[k];
+                if constexpr (kState) {
+                  tl[k] = in ? tr[k] : tl[k];
+                  walked[k] = in ? b + j + 1 : walked[k];
+                }
                 tr[k] = tr[k] * (1.f - e);
                 asum[k] = asum[k] + eo;
               }
@@ -401,10 +405,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     pix2rgb[3 * ip + 1] = cg[k];
     pix2rgb[3 * ip + 2] = cbl[k];
     if (pix2alpha) pix2alpha[ip] = asum[k];
-    if (pix2nwalk) pix2nwalk[ip] = walked[k];
-    if (pix2tlast) pix2tlast[ip] = tl[k];
+    if (kState && pix2nwalk) pix2nwalk[ip] = walked[k];
+    if (kState && pix2tlast) pix2tlast[ip] = tl[k];
   }
-  if (tile_cost) {
+  if (kState && tile_cost) {
     // The tile's longest walk: what its adjoint (and the next frame's
     // forward) will cost, used to schedule heavy tiles first. Every branch
     // here is on the (uniform) wave index and every lane of a wave stores

@@ -131,16 +131,18 @@ def _check_grad_acc(grad_acc, n, dev):
 
 
 def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0, tile_order=None,
-                  tile_cost=None):
+                  tile_cost=None, state=True):
     """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tlast [H,W]).
     tile_order (int32 [T], a permutation) schedules the tiles (results do not
-    depend on it); tile_cost (int32 [T]) receives each tile's longest walk."""
+    depend on it); tile_cost (int32 [T]) receives each tile's longest walk.
+    state=False: a plain forward, without the adjoint's state (nwalk and
+    tlast are returned as None; the kernel skips tracking them)."""
     dev = splat2.device
     w, h = img_shape
     img = torch.empty((h, w, 3), device=dev, dtype=torch.float32)
     alpha = torch.empty((h, w), device=dev, dtype=torch.float32)
-    nwalk = torch.empty((h, w), device=dev, dtype=torch.int32)
-    tlast = torch.empty((h, w), device=dev, dtype=torch.float32)
+    nwalk = torch.empty((h, w), device=dev, dtype=torch.int32) if state else None
+    tlast = torch.empty((h, w), device=dev, dtype=torch.float32) if state else None
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
     if tile_order is None and tile_cost is None:
         call("dsplat_gauss_rasterize_fwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),

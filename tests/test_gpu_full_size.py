@@ -53,6 +53,12 @@ def test_plain_forward_100k_512(oracle):
                          o_alpha, o_nwalk, self_flip=own)
     print(f"pairs {len(o_i2p)}, termination flips {flips}", flush=True)
     assert o_alpha.max() > 0.99
+    # the plain forward the workload times (RGB + alpha, no adjoint state), at
+    # every workgroup shape of 16x16 tiles: the same pixels, bit for bit
+    for thr in (64, 128, 256):
+        p_img, p_alpha, _, _ = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, thr, state=False)
+        np.testing.assert_array_equal(p_img.cpu().numpy(), img.cpu().numpy())
+        np.testing.assert_array_equal(p_alpha.cpu().numpy(), alpha.cpu().numpy())
 
 
 def test_batched_forward_2m_16_views():

@@ -276,6 +276,12 @@ def test_raster_parity(oracle, ts):
         flips = _check_pixels(g_img, g_alpha, g_nwalk.view(np.uint32), o_img, o_alpha, o_nwalk)
         print(f"ts={ts} threads={thr} threshold flips={flips}")
         np.testing.assert_allclose(g_img, img, atol=1e-6)
+        # the plain forward (no adjoint state tracked): the same pixels, bit for bit
+        p_img, p_alpha, none_w, none_t = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, thr,
+                                                          state=False)
+        assert none_w is None and none_t is None
+        np.testing.assert_array_equal(p_img.cpu().numpy(), g_img)
+        np.testing.assert_array_equal(p_alpha.cpu().numpy(), g_alpha)
     assert (o_alpha > 0.999).mean() > 0.05  # the scene does saturate pixels
 
 
