@@ -570,12 +570,15 @@ def train(args, torch, np, SG, scene):
     xyz, ls, q = params[:3 * n].view(n, 3), params[3 * n:6 * n].view(n, 3), params[6 * n:10 * n].view(n, 4)
     op = torch.sigmoid(params[10 * n:11 * n])
     rgb = params[11 * n:].view(n, 3)
-    # workgroup shape for this workload (the plan's forward and adjoint share
-    # it): the faster fused forward + adjoint over the first step's views
+    # tile size and workgroup shape for this workload (the plan's forward and
+    # adjoint share them): the fastest fused forward + adjoint over the first
+    # step's views. Pairs per (view, Gaussian) grow as tiles shrink: the
+    # capacity leaves ~1.7x room over the first step's count at every size.
+    pairs_per_item = {32: 14, 16: 14, 8: 40}
     tune = {}
-    for ts_c, th_c in ((32, 256), (32, 512)):
+    for ts_c, th_c in ((32, 256), (32, 512), (16, 64), (16, 128), (16, 256), (8, 64)):
         p = MultiViewPlan(n, views, ts_c, th_c, max_pixels=max_pix,
-                          pair_capacity=int(n * nview * 14))
+                          pair_capacity=int(n * nview * pairs_per_item[ts_c]))
         for rep in range(3):  # the first run warms up
             torch.cuda.synchronize()
             t1 = time.perf_counter()
@@ -583,6 +586,8 @@ def train(args, torch, np, SG, scene):
             p.backward(xyz, ls, q, op, out_rgb=o, target=targets)
             torch.cuda.synchronize()
             tune[f"{ts_c}/{th_c}"] = round((time.perf_counter() - t1) * 1000.0, 3)
+        if p.counters()[1]:
+            tune[f"{ts_c}/{th_c}"] = float("inf")  # overflowed: not a candidate
         p.close()
         del o
     ts, th = (int(v) for v in min(tune, key=tune.get).split("/"))
@@ -590,7 +595,7 @@ def train(args, torch, np, SG, scene):
     # gradient agreement on step 0: batched adjoint vs the sum over views of
     # the single-view adjoint (both before any update)
     plan = MultiViewPlan(n, views, ts, th, max_pixels=max_pix,
-                         pair_capacity=int(n * nview * 14))
+                         pair_capacity=int(n * nview * pairs_per_item[ts]))
     out, _ = plan.forward(xyz, ls, q, op, rgb)
     k0, over = plan.counters()
     if over:
