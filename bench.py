@@ -482,10 +482,11 @@ def batched(args, torch, np, SG, scene):
         tflat = torch.cat([x.reshape(-1) for x in targets])
         mats = [t(mm) for mm, _ in views]
 
-        # the per-view path runs with the single-view autotuned schedule
-        pts, pth, pthb = (args.tile or 16), args.threads, getattr(args, "threads_bwd", 0)
-
-        def per_view():
+        # the per-view path: the single-view pipeline per view, at the tile
+        # size and workgroup sizes that are fastest for it on this workload
+        # (tile, forward threads, adjoint threads), measured like the plan's
+        def per_view(cfg):
+            pts, pth, pthb = cfg
             for v, (_, shape) in enumerate(views):
                 pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, mats[v], shape)
                 t2i, i2p = SG.bin_gauss(shape, pts, pxd, aabb)
@@ -496,13 +497,21 @@ def batched(args, torch, np, SG, scene):
                                                              tlast, d_img, None, pthb)
                     SG.project_bwd(xyz, ls, q, mats[v], shape, dpix, dcon)
 
-        per_view()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            per_view()
-        torch.cuda.synchronize()
-        pv = args.views * reps / (time.perf_counter() - t0)
+        def per_view_rate(cfg, nrep):
+            per_view(cfg)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(nrep):
+                per_view(cfg)
+            torch.cuda.synchronize()
+            return args.views * nrep / (time.perf_counter() - t0)
+
+        pv_tune = {}
+        for cfg in ((8, 64, 64), (16, 128, 128), (16, 256, 256), (32, 256, 256), (32, 512, 256),
+                    (32, 1024, 512)):
+            pv_tune["/".join(map(str, cfg))] = round(per_view_rate(cfg, 1), 2)
+        pv_cfg = tuple(int(x) for x in max(pv_tune, key=pv_tune.get).split("/"))
+        pv = per_view_rate(pv_cfg, reps)
 
         (ts, th), tune = autotune.tune_multiview(n, views, (xyz, ls, q, op, rgb), mode=mode,
                                                  target=tflat, reps=2)
@@ -536,7 +545,9 @@ def batched(args, torch, np, SG, scene):
                      "resolutions": [f"{w}x{h}" for _, (w, h) in views],
                      "per_view_views_per_s": round(pv, 2), "batched_views_per_s": round(bv, 2),
                      "speedup": round(bv / pv, 3), "tile": ts, "threads": th, "pairs_K": k,
-                     "autotune_ms": {kk: round(vv, 3) for kk, vv in tune.items()}}
+                     "autotune_ms": {kk: round(vv, 3) for kk, vv in tune.items()},
+                     "per_view_config": "/".join(map(str, pv_cfg)),
+                     "per_view_tune_views_per_s": pv_tune}
     return out
 
 
