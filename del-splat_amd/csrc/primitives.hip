@@ -163,9 +163,13 @@ int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t*
 // the digit's total), scatter (digit bases from the totals, then the stable
 // in-tile rank).
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 8;                        // rounds per wave
-constexpr int kSortWaveChunk = 64 * kSortItems;      // 512
-constexpr int kSortTile = kSortBlock * kSortItems;   // 2048
+// Rounds per wave: 8 (2048-element tiles), or 16 (4096) for sorts of at
+// least kBigSort elements, whose scatter then writes runs twice as long per
+// digit (fewer partial lines) over tiles that still fill the GPU.
+constexpr int kSortItems = 8;
+constexpr int kSortItemsBig = 16;
+constexpr uint32_t kBigSort = 8u << 20;
+constexpr int kSortTile = kSortBlock * kSortItems;   // the smaller tile (sizes the workspace)
 constexpr int kSortWaves = kSortBlock / 64;
 
 struct SortLayout {
@@ -206,13 +210,14 @@ SortLayout sort_layout(void* base, uint32_t n) {
   return L;
 }
 
-template <typename KeyT>
+template <typename KeyT, int ITEMS>
 __global__ void __launch_bounds__(kSortBlock)
 radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ keys,
                      uint32_t shift, uint32_t dbits, uint32_t nb, uint32_t* __restrict__ cnt) {
+  constexpr int kTile = kSortBlock * ITEMS;
   __shared__ uint32_t h[256];
   const uint32_t n = load_n(n_cap, d_n);
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+  const uint64_t base = uint64_t(blockIdx.x) * kTile;
   // tiles past n (the grid covers the capacity) have nothing to count: the
   // column scan and the scatter stop at the last tile holding an element
   if (base >= n) return;  // uniform
@@ -221,14 +226,14 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
   // all loads first, then the histogram
-  KeyT kk[kSortItems];
+  KeyT kk[ITEMS];
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
     kk[k] = i < n ? keys[i] : KeyT(0);
   }
 #pragma unroll
-  for (int k = 0; k < kSortItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const uint64_t i = base + uint64_t(k) * kSortBlock + threadIdx.x;
     const bool valid = i < n;
     const uint32_t d = valid ? uint32_t(kk[k] >> shift) & mask : 0u;
@@ -251,25 +256,26 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
 // Workgroup d: cnt[d][b] <- elements of digit d in tiles before b;
 // tot[d] <- elements of digit d.
 __global__ void __launch_bounds__(kSortBlock)
-radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
+radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb, uint32_t tile,
                      uint32_t* __restrict__ cnt, uint32_t* __restrict__ tot) {
   const uint32_t n = load_n(n_cap, d_n);
-  const uint32_t nbv = div_up(n, kSortTile);
+  const uint32_t nbv = div_up(n, tile);
   const uint32_t d = blockIdx.x;
   uint32_t carry = 0;
   uint32_t* col = cnt + size_t(d) * col_stride(nb);
-  for (uint32_t b0 = 0; b0 < nbv; b0 += kSortTile) {
-    const uint32_t i0 = b0 + threadIdx.x * kSortItems;
-    uint32_t v[kSortItems], s = 0;
+  constexpr int kPer = 8, kSpan = kSortBlock * kPer;  // column values per thread and round
+  for (uint32_t b0 = 0; b0 < nbv; b0 += kSpan) {
+    const uint32_t i0 = b0 + threadIdx.x * kPer;
+    uint32_t v[kPer], s = 0;
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < kPer; ++k) {
       v[k] = i0 + k < nbv ? col[col_slot(i0 + k, nb)] : 0u;
       s += v[k];
     }
     uint32_t chunk;
     uint32_t run = carry + block_exclusive_scan(s, &chunk);
 #pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
+    for (int k = 0; k < kPer; ++k) {
       if (i0 + k < nbv) col[col_slot(i0 + k, nb)] = run;
       run += v[k];
     }
@@ -281,7 +287,7 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb,
 // iota: the values are the element indices (vin is not read). cdst (may be
 // null): also gathers rsrc[value], a packed tile rectangle, writing it to
 // rdst[pos] and its area to cdst[pos].
-template <typename KeyT>
+template <typename KeyT, int ITEMS>
 __global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
                      const uint32_t* __restrict__ vin, bool iota, KeyT* __restrict__ kout,
@@ -291,13 +297,14 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
                      const uint2* __restrict__ rsrc, uint32_t* __restrict__ cdst,
                      uint2* __restrict__ rdst) {
   static_assert(kSortBlock == 256, "one thread per digit");
+  constexpr int kTile = kSortBlock * ITEMS, kWaveChunk = 64 * ITEMS;
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dpre[256];
   __shared__ uint32_t dbase[256];
-  __shared__ KeyT skey[kSortTile];
-  __shared__ uint32_t sval[kSortTile];
+  __shared__ KeyT skey[kTile];
+  __shared__ uint32_t sval[kTile];
   const uint32_t n = load_n(n_cap, d_n);
-  const uint64_t bstart = uint64_t(blockIdx.x) * kSortTile;
+  const uint64_t bstart = uint64_t(blockIdx.x) * kTile;
   if (bstart >= n) return;  // uniform
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // the digit total and this tile's column prefix of digit threadIdx.x,
@@ -309,18 +316,18 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
-  const uint64_t base = bstart + uint64_t(w) * kSortWaveChunk;
-  KeyT key[kSortItems];
-  uint32_t val[kSortItems], rank[kSortItems], dig[kSortItems];
+  const uint64_t base = bstart + uint64_t(w) * kWaveChunk;
+  KeyT key[ITEMS];
+  uint32_t val[ITEMS], rank[ITEMS], dig[ITEMS];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint64_t i = base + uint64_t(r) * 64 + lane;
     const bool valid = i < n;
     key[r] = valid ? kin[i] : KeyT(0);
     val[r] = valid ? (iota ? uint32_t(i) : vin[i]) : 0u;
   }
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint64_t i = base + uint64_t(r) * 64 + lane;
     const bool valid = i < n;
     const uint32_t d = uint32_t(key[r] >> shift) & mask;
@@ -357,7 +364,7 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   __syncthreads();
   // stage in LDS in tile-sorted order ...
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     if (dig[r] == 0xffffffffu) continue;
     const uint32_t d = dig[r];
     const uint32_t lp = dpre[d] + wcnt[w][d] + rank[r];
@@ -367,15 +374,15 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   __syncthreads();
   // ... and write each digit's run to consecutive global addresses (the
   // gathered values loaded for all rounds before any is stored)
-  const uint32_t nvalid = uint32_t(min(uint64_t(kSortTile), n - bstart));
-  uint2 gv[kSortItems];
+  const uint32_t nvalid = uint32_t(min(uint64_t(kTile), n - bstart));
+  uint2 gv[ITEMS];
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
     gv[r] = (cdst && i < nvalid) ? rsrc[sval[i]] : make_uint2(0u, 0u);
   }
 #pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
     if (i < nvalid) {
       const KeyT k = skey[i];
@@ -412,7 +419,9 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
   if (keys_alt_out) *keys_alt_out = kalt;
   if (vals_alt_out) *vals_alt_out = L.valt;
   if (n_cap == 0) return DSPLAT_OK;
-  const uint32_t nb = div_up(n_cap, kSortTile);
+  const bool big = n_cap >= kBigSort;
+  const uint32_t tile = kSortBlock * (big ? kSortItemsBig : kSortItems);
+  const uint32_t nb = div_up(n_cap, tile);
   // as few passes as 8-bit digits allow, with the bits spread evenly over
   // them: fewer digits per pass give longer runs per digit in the scatter
   // (11 bits: 6 + 5 rather than 8 + 3)
@@ -422,13 +431,21 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
   uint32_t shift = 0;
   for (uint32_t p = 0; p < P; ++p) {
     const uint32_t dbits = div_up(key_bits - shift, P - p);
-    radix_upsweep_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb,
-                                                         L.cnt);
-    radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, L.cnt, L.tot);
     const bool last = p + 1 == P;
-    radix_scatter_kernel<KeyT><<<nb, kSortBlock, 0, s>>>(
-        n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt, L.tot,
-        1u << dbits, rect_src, last ? cnt_dst : nullptr, last ? rect_dst : nullptr);
+#define DSPLAT_PASS(I_)                                                                        \
+  do {                                                                                         \
+    radix_upsweep_kernel<KeyT, I_><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb, \
+                                                             L.cnt);                           \
+    radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, tile, L.cnt, L.tot); \
+    radix_scatter_kernel<KeyT, I_><<<nb, kSortBlock, 0, s>>>(                                  \
+        n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt,      \
+        L.tot, 1u << dbits, rect_src, last ? cnt_dst : nullptr, last ? rect_dst : nullptr);    \
+  } while (0)
+    if (big)
+      DSPLAT_PASS(kSortItemsBig);
+    else
+      DSPLAT_PASS(kSortItems);
+#undef DSPLAT_PASS
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;

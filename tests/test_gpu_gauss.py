@@ -238,6 +238,26 @@ def test_radix_sort_large_repeated():
         np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), order.astype(np.uint32))
 
 
+@pytest.mark.parametrize("n,bits", [(9_000_017, 37), (8 << 20, 15)])
+def test_radix_sort_big_tiles(n, bits):
+    """Sorts of at least 8M elements take 4096-element tiles: stable against
+    numpy, with ties, in a dirty workspace, at the size boundary too."""
+    from del_splat_dlpack import _native
+    L = _native.lib()
+    wsb = L.dsplat_radix_sort_workspace_size(n)
+    ws = torch.full((wsb,), 0x5A, device=DEV, dtype=torch.uint8)
+    rng = np.random.default_rng(n + bits)
+    keys = rng.integers(0, 1 << bits, n, dtype=np.uint64)
+    keys[rng.integers(0, n, n // 4)] = keys[7]
+    k = torch.from_numpy(keys.view(np.int64).copy()).to(DEV)
+    v = torch.arange(n, device=DEV, dtype=torch.int32)
+    _native.call("dsplat_radix_sort_by_key_u64_u32", torch.cuda.current_stream().cuda_stream, n,
+                 k.data_ptr(), v.data_ptr(), bits, ws.data_ptr(), wsb)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(k.cpu().numpy().view(np.uint64), keys[order])
+    np.testing.assert_array_equal(v.cpu().numpy().view(np.uint32), order.astype(np.uint32))
+
+
 @pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 1_000_001])
 def test_exclusive_scan(n):
     from del_splat_dlpack import _native
