@@ -245,6 +245,32 @@ __device__ __forceinline__ EntryHits<TS, NT> entry_hits(const EntryMasks<TS, NT,
   return h;
 }
 
+// The entries j < cnt of a batch whose box reaches one of the wave's groups
+// (regions first .. first + G - 1), as a bitmask over the batch: lane l of
+// the wave tests entries l and l + 64. The entry loops walk only the set
+// bits, so an entry that misses all of the wave's pixels costs it nothing
+// (in dense scenes most entries of a tile's list miss most of its waves).
+template <int TS, int NT, int B>
+__device__ __forceinline__ void wave_entry_mask(const EntryMasks<TS, NT, B>& m, uint32_t cnt,
+                                                uint32_t first, uint64_t (&em)[(B + 63) / 64]) {
+  using GM = GroupMap<TS, NT>;
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int h = 0; h < (B + 63) / 64; ++h) {
+    const uint32_t j = uint32_t(h) * 64u + lane;
+    bool reach = false;
+    if (j < cnt) {
+      if constexpr (GM::kLaneMask) {
+#pragma unroll
+        for (int g = 0; g < GM::G; ++g) reach = reach || m.lanes[(first + uint32_t(g)) * B + j] != 0;
+      } else {
+        reach = ((m.region[j] >> first) & ((1u << GM::G) - 1u)) != 0u;
+      }
+    }
+    em[h] = __ballot(reach);
+  }
+}
+
 // Lane-mask form (one pixel per lane): the lanes of group g that take the
 // entry are the walking lanes (a ballot) inside the box (the group's lane
 // mask), combined in scalar registers. Returns false when there are none
@@ -339,7 +365,18 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     // (x + 0 = x, T * 1 = T); only wave-uniform skips branch.
     // No continue / break in this loop: jumps out of a scope with locals
     // were compiled into a per-entry exit-state switch in scalar code.
-    for (uint32_t j = 0; j < cnt; ++j) {
+    // Only the entries that reach one of the wave's groups are visited, in
+    // list order (the set bits of the wave's entry mask, lowest first).
+    constexpr int kEW = (B + 63) / 64;
+    uint64_t em[kEW];
+    wave_entry_mask<TS, NT, B>(smask, cnt, gfirst, em);
+    bool batch_done = false;
+#pragma unroll
+    for (int h = 0; h < kEW; ++h) {
+    uint64_t pending = batch_done ? 0ull : em[h];
+    while (pending != 0) {
+      const uint32_t j = uint32_t(h) * 64u + uint32_t(__builtin_ctzll(pending));
+      pending &= pending - 1;
       // groups of this wave that the box reaches (scalar)
       const EntryHits<TS, NT> eh = entry_hits(smask, j, gfirst);
       const uint32_t hits = eh.hits;
@@ -354,7 +391,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           anylive |= live_lanes[k];
         }
         if (anylive == 0) {
-          j = cnt;  // every pixel of the wave is done: leave the batch
+          // every pixel of the wave is done: leave the batch
+          pending = 0;
+          batch_done = true;
         } else {
           const float4 q2 = s2[j];
           const float4 q3 = s3[j];
@@ -390,6 +429,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           }
         }
       }
+    }
     }
     bool lane_live = false;
 #pragma unroll
@@ -653,9 +693,24 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       for (int c = 0; c < kGrad; ++c) acc[c][threadIdx.x] = 0.f;
     }
     __syncthreads();
-    for (uint32_t j = 0; j < cnt; ++j) {
+    // the entries that reach one of the wave's groups, minus those beyond
+    // every walk of the wave (slot j holds walk position hi - 1 - j, so
+    // those are the slots below hi - wmax)
+    constexpr int kEW = (B + 63) / 64;
+    uint64_t em[kEW];
+    wave_entry_mask<TS, NT, B>(smask, cnt, gfirst, em);
+    const uint32_t jmin = hi > wmax ? hi - wmax : 0u;
+#pragma unroll
+    for (int h = 0; h < kEW; ++h) {
+      const uint32_t j0 = uint32_t(h) * 64u;
+      if (jmin >= j0 + 64u) em[h] = 0;
+      else if (jmin > j0) em[h] &= ~0ull << (jmin - j0);
+    }
+#pragma unroll
+    for (int h = 0; h < kEW; ++h)
+    for (uint64_t pending = em[h]; pending != 0; pending &= pending - 1) {
+      const uint32_t j = uint32_t(h) * 64u + uint32_t(__builtin_ctzll(pending));
       const uint32_t it = hi - 1 - j;
-      if (it >= wmax) continue;  // beyond every walk of the wave
       // groups of this wave that the box reaches (scalar)
       const EntryHits<TS, NT> eh = entry_hits(smask, j, gfirst);
       const uint32_t hits = eh.hits;
