@@ -180,15 +180,18 @@ struct SortLayout {
   size_t total;
 };
 
-// Column slot of sort tile b among nb: workgroups are dispatched to the 8
-// XCDs round-robin, so the tiles of one XCD (b mod 8 alike) are made
-// adjacent. A column's words written by the upsweep then share cache lines
-// only with words written on the same XCD and merge in its L2; in tile
-// order every line took one word from each of the eight XCDs.
+// Sort tile of workgroup b in a pass over nbv tiles. Workgroups are
+// dispatched to the 8 XCDs round-robin (b mod 8), and each XCD takes a
+// contiguous range of tiles: consecutive tiles then run on the same XCD at
+// about the same time, so the runs that they write side by side for one
+// digit (and their words of a count column) meet in one L2 and leave it as
+// whole lines. With tile b on workgroup b, every digit run shared its end
+// lines with a run written through another XCD's L2.
 constexpr uint32_t kXcds = 8;
 __host__ __device__ inline uint32_t col_stride(uint32_t nb) { return kXcds * div_up(nb, kXcds); }
-__device__ inline uint32_t col_slot(uint32_t b, uint32_t nb) {
-  return (b % kXcds) * div_up(nb, kXcds) + b / kXcds;
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t nbv) {
+  const uint32_t x = b % kXcds, q = nbv / kXcds, r = nbv % kXcds;
+  return x * q + min(x, r) + b / kXcds;
 }
 
 template <typename KeyT>
@@ -217,10 +220,13 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   constexpr int kTile = kSortBlock * ITEMS;
   __shared__ uint32_t h[256];
   const uint32_t n = load_n(n_cap, d_n);
-  const uint64_t base = uint64_t(blockIdx.x) * kTile;
-  // tiles past n (the grid covers the capacity) have nothing to count: the
-  // column scan and the scatter stop at the last tile holding an element
-  if (base >= n) return;  // uniform
+  const uint32_t nbv = div_up(n, uint32_t(kTile));
+  // workgroups past the tiles holding elements (the grid covers the
+  // capacity) have nothing to count: the column scan and the scatter stop
+  // at the last tile holding an element
+  if (blockIdx.x >= nbv) return;  // uniform
+  const uint32_t tb = xcd_tile(blockIdx.x, nbv);
+  const uint64_t base = uint64_t(tb) * kTile;
   const int lane = threadIdx.x & 63;
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -250,7 +256,7 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   }
   __syncthreads();
   if (threadIdx.x <= mask)
-    cnt[size_t(threadIdx.x) * col_stride(nb) + col_slot(blockIdx.x, nb)] = h[threadIdx.x];
+    cnt[size_t(threadIdx.x) * col_stride(nb) + tb] = h[threadIdx.x];
 }
 
 // Workgroup d: cnt[d][b] <- elements of digit d in tiles before b;
@@ -263,21 +269,38 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb, uint32_t 
   const uint32_t d = blockIdx.x;
   uint32_t carry = 0;
   uint32_t* col = cnt + size_t(d) * col_stride(nb);
-  constexpr int kPer = 8, kSpan = kSortBlock * kPer;  // column values per thread and round
+  // column values per thread and round, as two 16-byte words (the column
+  // starts on a multiple of 8 words and holds col_stride(nb) >= nbv of them)
+  constexpr int kPer = 8, kSpan = kSortBlock * kPer;
   for (uint32_t b0 = 0; b0 < nbv; b0 += kSpan) {
     const uint32_t i0 = b0 + threadIdx.x * kPer;
+    const bool full = i0 + kPer <= nbv;
     uint32_t v[kPer], s = 0;
+    if (full) {
+      const uint4 a = *reinterpret_cast<const uint4*>(col + i0);
+      const uint4 b = *reinterpret_cast<const uint4*>(col + i0 + 4);
+      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    } else {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      v[k] = i0 + k < nbv ? col[col_slot(i0 + k, nb)] : 0u;
-      s += v[k];
+      for (int k = 0; k < kPer; ++k) v[k] = i0 + k < nbv ? col[i0 + k] : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) s += v[k];
     uint32_t chunk;
     uint32_t run = carry + block_exclusive_scan(s, &chunk);
+    uint32_t o[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      if (i0 + k < nbv) col[col_slot(i0 + k, nb)] = run;
+      o[k] = run;
       run += v[k];
+    }
+    if (full) {
+      *reinterpret_cast<uint4*>(col + i0) = make_uint4(o[0], o[1], o[2], o[3]);
+      *reinterpret_cast<uint4*>(col + i0 + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k)
+        if (i0 + k < nbv) col[i0 + k] = o[k];
     }
     carry += chunk;
   }
@@ -304,15 +327,17 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   __shared__ KeyT skey[kTile];
   __shared__ uint32_t sval[kTile];
   const uint32_t n = load_n(n_cap, d_n);
-  const uint64_t bstart = uint64_t(blockIdx.x) * kTile;
-  if (bstart >= n) return;  // uniform
+  const uint32_t nbv = div_up(n, uint32_t(kTile));
+  if (blockIdx.x >= nbv) return;  // uniform
+  const uint32_t tb = xcd_tile(blockIdx.x, nbv);
+  const uint64_t bstart = uint64_t(tb) * kTile;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // the digit total and this tile's column prefix of digit threadIdx.x,
   // fetched now so their latency hides behind the ranking
   const bool live_digit = threadIdx.x < nbins;
   const uint32_t my_tot = live_digit ? tot[threadIdx.x] : 0u;
   const uint32_t my_off =
-      live_digit ? off[size_t(threadIdx.x) * col_stride(nb) + col_slot(blockIdx.x, nb)] : 0u;
+      live_digit ? off[size_t(threadIdx.x) * col_stride(nb) + tb] : 0u;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
   __syncthreads();
   const uint32_t mask = (1u << dbits) - 1u;
