@@ -344,11 +344,18 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     tr[k] = tl[k] = 1.f;
     walked[k] = 0;
   }
+  // The staging threads hold the next batch's records in registers, loaded
+  // while the current batch is walked, and its list indices one batch
+  // earlier still (the record load depends on them).
+  const bool stager = threadIdx.x < uint32_t(B);
+  float4 n0, n1, n2, n3;
+  uint32_t nidx = 0;
+  if (stager && threadIdx.x < num) ld.load(idx2pnt[end - 1 - threadIdx.x], n0, n1, n2, n3);
+  if (stager && uint32_t(B) + threadIdx.x < num) nidx = idx2pnt[end - 1 - (uint32_t(B) + threadIdx.x)];
   for (uint32_t b = 0; b < num; b += B) {
     const uint32_t it = b + threadIdx.x;
-    if (threadIdx.x < uint32_t(B) && it < num) {
-      float4 q0, q1, q2, q3;
-      ld.load(idx2pnt[end - 1 - it], q0, q1, q2, q3);
+    if (stager && it < num) {
+      const float4 q0 = n0, q1 = n1, q2 = n2, q3 = n3;
       // the conic with b doubled once per entry (every wave uses it)
       const ConicQ ke = conic_q(q0.z, q0.w, q1.x);
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
@@ -358,6 +365,8 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       smask.stage(threadIdx.x, tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     __syncthreads();
+    if (stager && it + uint32_t(B) < num) ld.load(nidx, n0, n1, n2, n3);
+    if (stager && it + uint32_t(2 * B) < num) nidx = idx2pnt[end - 1 - (it + uint32_t(2 * B))];
     // scalar (a min formed as v_med3 would put the loop bound in a VGPR)
     const uint32_t cnt = __builtin_amdgcn_readfirstlane(min(uint32_t(B), num - b));
     // Branch-free per pixel: a splat that misses the pixel (or a finished
