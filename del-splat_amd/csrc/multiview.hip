@@ -46,6 +46,10 @@ struct ViewDev {
 };
 
 constexpr uint32_t kMaxViews = 64;
+static_assert(kMaxViews <= kSortMaxSegs, "one depth-sort segment per view");
+// views of at least this many Gaussians are depth-sorted one by one (below,
+// the padding to whole segments costs more than the locality saves)
+constexpr uint32_t kSegSortMin = 1u << 18;
 
 // Finds the view of a global tile (views are few; the scan is uniform across
 // the workgroup).
@@ -61,7 +65,7 @@ __device__ inline float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
 // ---------------------------------------------------------- projection ---
 template <bool kLogScale, bool kLogitOp, bool kRgbItem>
 __global__ void __launch_bounds__(256)
-mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
+mv_project_kernel(uint32_t n, uint32_t seg, const ViewDev* __restrict__ views, uint32_t ts,
                   const float* __restrict__ xyz, const float* __restrict__ scale,
                   const float4* __restrict__ quat, const float* __restrict__ opacity,
                   const float* __restrict__ rgb, float4* __restrict__ rec,
@@ -72,8 +76,15 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wbase = threadIdx.x & ~63u;
   const uint32_t g0 = blockIdx.x * blockDim.x + wbase;
-  if (g0 >= n) return;  // wave-uniform
   const uint32_t v = blockIdx.y;
+  // the view's depth-sort segment holds its n items and seg - n padding
+  // entries: an empty rectangle (no pairs) under the largest key
+  const size_t sp = size_t(v) * seg + i;
+  if (i >= n && i < seg) {
+    rect[sp] = make_uint2(0u, 0u);
+    dkey[sp] = 0x7FFFFFFFu;
+  }
+  if (g0 >= n) return;  // wave-uniform
   const ViewDev& vd = views[v];
   if (i < n) {
     const float4 q = quat[i];
@@ -96,8 +107,8 @@ mv_project_kernel(uint32_t n, const ViewDev* __restrict__ views, uint32_t ts,
     mine[2] = make_float4(col[2], p.aabb[0], p.aabb[1], p.aabb[2]);
     mine[3] = make_float4(p.aabb[3], 0.f, 0.f, 0.f);
     const TileRect tr = tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], ts, vd.tw, vd.th);
-    rect[item] = pack_rect(tr);
-    dkey[item] = depth_key(p.ndcz);
+    rect[sp] = pack_rect(tr);
+    dkey[sp] = depth_key(p.ndcz);
   }
   wave_lds_fence();  // every lane of the wave is past its LDS writes
   const uint32_t nrec4 = 4u * min(64u, n - g0);
@@ -205,10 +216,12 @@ __global__ void mv_clamp_kernel(uint32_t* __restrict__ misc, uint32_t capacity) 
   misc[4] = k > capacity ? 1u : 0u;
 }
 
-// First pair slot of item j (depth order): the chunk prefix of its
-// workgroup's 256 items plus a block scan of the counts cnt.
+// First pair slot of entry j (depth order): the chunk prefix of its
+// workgroup's 256 entries plus a block scan of the counts cnt. The sorted
+// values are positions in the depth sort's segments of seg entries per view
+// (item g of view v at v seg + g); padding entries have no pairs.
 __global__ void __launch_bounds__(kScanBlock)
-mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views,
+mv_fill_kernel(uint32_t nitems, uint32_t n, uint32_t seg, const ViewDev* __restrict__ views,
                const uint32_t* __restrict__ order, const uint2* __restrict__ rect_sorted,
                const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ chunk_prefix,
                const uint32_t* __restrict__ misc,
@@ -220,8 +233,9 @@ mv_fill_kernel(uint32_t nitems, uint32_t n, const ViewDev* __restrict__ views,
   uint32_t item = 0, c = 0, tbase = 0, tw = 1;
   TileRect r{0, 0, 0, 0};
   if (valid) {
-    item = order[j];
-    const ViewDev& vd = views[item / n];
+    const uint32_t pos = order[j], v = pos / seg;
+    item = v * n + (pos - v * seg);
+    const ViewDev& vd = views[v];
     r = unpack_rect(rect_sorted[j]);  // gathered into depth order by the sort
     c = cnt[j];
     tbase = vd.tile_off;
@@ -417,6 +431,9 @@ using namespace dsplat;
 // ----------------------------------------------------------------- plan ---
 struct dsplat_mv_plan {
   uint32_t n = 0, nview = 0, max_view = 0, ts = 16, threads = 0, capacity = 0;
+  // entries per view in the depth sort: n, or n padded to whole sort
+  // segments when the views are sorted each on its own
+  uint32_t seg = 0;
   uint32_t max_tiles = 0, total_tiles = 0;
   uint64_t max_pix = 0, total_pix = 0;
   uint32_t persist_fwd = 0, persist_bwd = 0;
@@ -425,7 +442,9 @@ struct dsplat_mv_plan {
   size_t mem_bytes = 0;
   ViewDev* d_views = nullptr;
   float4* rec = nullptr;
-  uint2 *rect = nullptr, *rect_sorted = nullptr;  // tile rectangles per item, and in depth order
+  // depth sort (seg entries per view): tile rectangles and keys per entry,
+  // the sorted entries, and their rectangles and pair counts in depth order
+  uint2 *rect = nullptr, *rect_sorted = nullptr;
   uint32_t *dkey = nullptr, *order = nullptr;
   void* dsort_ws = nullptr;
   size_t dsort_bytes = 0;
@@ -548,16 +567,30 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
                          uint64_t(max_view) * (2 * 4096 / tile_size + 1);
   P->max_tiles = uint32_t(tiles > 0xFFFFFFF0ull ? 0xFFFFFFF0ull : tiles);
   const size_t items = size_t(num_pnt) * max_view;
+  // Views of at least kSegSortMin Gaussians are depth-sorted each in its own
+  // segment (padded to whole segments): the sort's last pass gathers every
+  // entry's tile rectangle, and within one view's segment those reads stay
+  // in a slice of the rectangles small enough for the caches, where across
+  // all views they were random reads over the whole array.
+  const uint64_t seg = num_pnt >= kSegSortMin
+                           ? (uint64_t(num_pnt) + kSortSegAlign - 1) / kSortSegAlign * kSortSegAlign
+                           : num_pnt;
+  if (seg * max_view > 0xFFFFFFFFull) {
+    delete P;
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "multiview: too many items for the depth sort");
+  }
+  P->seg = uint32_t(seg);
+  const size_t sitems = size_t(seg) * max_view;
   size_t o = 0;
   auto take = [&](size_t b) { size_t r = o; o += a256(b); return r; };
   const size_t o_views = take(sizeof(ViewDev) * max_view);
   const size_t o_rec = take(items * 64);
-  const size_t o_rect = take(items * 8), o_rsorted = take(items * 8);
-  const size_t o_dkey = take(items * 4), o_order = take(items * 4);
-  P->dsort_bytes = sort_workspace_bytes<uint32_t>(uint32_t(items));
+  const size_t o_rect = take(sitems * 8), o_rsorted = take(sitems * 8);
+  const size_t o_dkey = take(sitems * 4), o_order = take(sitems * 4);
+  P->dsort_bytes = sort_workspace_bytes<uint32_t>(uint32_t(sitems));
   const size_t o_dsort = take(P->dsort_bytes);
-  const size_t o_off = take((items + 1) * 4);
-  const size_t o_scan = take(scan_chunks_workspace_bytes(uint32_t(items)));
+  const size_t o_off = take((sitems + 1) * 4);
+  const size_t o_scan = take(scan_chunks_workspace_bytes(uint32_t(sitems)));
   const size_t o_tkey = take(size_t(pair_capacity) * 4 + 4);
   const size_t o_i2p = take(size_t(pair_capacity) * 4 + 4);
   P->tsort_bytes = sort_workspace_bytes<uint32_t>(pair_capacity);
@@ -798,9 +831,9 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   P->fwd_valid = false;
   if (!out_alpha) out_alpha = P->alpha;
   const uint32_t n = P->n, nv = P->nview;
-  const uint32_t items = n * nv;
   const bool log_scale = flags & DSPLAT_SCALE_IS_LOG, logit = flags & DSPLAT_OPACITY_IS_LOGIT;
-  const dim3 pg(div_up(n, 256), nv);
+  const uint32_t sitems = P->seg * nv;  // depth-sort entries
+  const dim3 pg(div_up(P->seg, 256), nv);
   const float4* q4 = reinterpret_cast<const float4*>(quat);
   const float* col = rgb;
   if (P->sh_k) {
@@ -816,8 +849,8 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
   P->fwd_sh_k = P->sh_k;
   P->fwd_sh_deg = P->sh_deg;
 #define DSPLAT_MVP(A, B, C)                                                                    \
-  mv_project_kernel<A, B, C><<<pg, 256, 0, s>>>(n, P->d_views, P->ts, xyz, scale, q4, opacity, \
-                                                col, P->rec, P->rect, P->dkey)
+  mv_project_kernel<A, B, C><<<pg, 256, 0, s>>>(n, P->seg, P->d_views, P->ts, xyz, scale, q4,  \
+                                                opacity, col, P->rec, P->rect, P->dkey)
 #define DSPLAT_MVP2(A, B) \
   if (P->sh_k) DSPLAT_MVP(A, B, true); else DSPLAT_MVP(A, B, false)
   if (log_scale && logit) { DSPLAT_MVP2(true, true); }
@@ -828,20 +861,22 @@ static int mv_forward(dsplat_mv_plan* P, hipStream_t s, const float* xyz, const 
 #undef DSPLAT_MVP
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
-  // values: list items (iota), generated by the first pass; the last pass
-  // also gathers the tile rectangles and their areas (the pair counts) into
-  // depth order
-  int rc = radix_sort_pairs<uint32_t>(s, items, nullptr, P->dkey, P->order, 31, P->dsort_ws,
+  // values: entry positions (iota), generated by the first pass; the last
+  // pass also gathers the tile rectangles and their areas (the pair counts)
+  // into depth order. Large views are sorted each in its own segment: the
+  // tile lists come out the same (every tile belongs to one view, and within
+  // a view the order is by depth, then item, either way).
+  int rc = radix_sort_pairs<uint32_t>(s, sitems, nullptr, P->dkey, P->order, 31, P->dsort_ws,
                                       P->dsort_bytes, &in_alt, &kalt, &valt, true, P->rect,
-                                      P->off, P->rect_sorted);
+                                      P->off, P->rect_sorted, n >= kSegSortMin ? P->seg : 0u);
   if (rc) return rc;
   const uint32_t* order = in_alt ? valt : P->order;
-  rc = scan_chunk_prefixes(s, items, P->off, P->scan_ws, P->misc);
+  rc = scan_chunk_prefixes(s, sitems, P->off, P->scan_ws, P->misc);
   if (rc) return rc;
   mv_clamp_kernel<<<1, 1, 0, s>>>(P->misc, P->capacity);
-  mv_fill_kernel<<<div_up(items, kScanBlock), kScanBlock, 0, s>>>(
-      items, n, P->d_views, order, P->rect_sorted, P->off, P->scan_ws, P->misc, P->tilekey,
-      P->idx2pnt);
+  mv_fill_kernel<<<div_up(sitems, kScanBlock), kScanBlock, 0, s>>>(
+      sitems, n, P->seg, P->d_views, order, P->rect_sorted, P->off, P->scan_ws, P->misc,
+      P->tilekey, P->idx2pnt);
   const uint32_t* tsorted = P->tilekey;
   if (P->capacity > 0 && P->total_tiles > 1) {
     rc = radix_sort_pairs<uint32_t>(s, P->capacity, P->misc + 1, P->tilekey, P->idx2pnt,

@@ -194,6 +194,17 @@ __device__ inline uint32_t xcd_tile(uint32_t b, uint32_t nbv) {
   return x * q + min(x, r) + b / kXcds;
 }
 
+// Sort tile of workgroup b when the nbv tiles form segments of cps tiles
+// each (cps = 0: one segment): segment after segment in workgroup order, and
+// within a segment (cps a multiple of 8, so b mod 8 is still the XCD) each
+// XCD takes a contiguous range. The workgroups resident at once then work on
+// one or two segments.
+__device__ inline uint32_t seg_tile(uint32_t b, uint32_t nbv, uint32_t cps) {
+  if (cps == 0) return xcd_tile(b, nbv);
+  const uint32_t sg = b / cps;
+  return sg * cps + xcd_tile(b - sg * cps, cps);
+}
+
 template <typename KeyT>
 SortLayout sort_layout(void* base, uint32_t n) {
   const uint32_t nb = div_up(n ? n : 1, kSortTile);
@@ -207,7 +218,7 @@ SortLayout sort_layout(void* base, uint32_t n) {
   };
   L.kalt = take(size_t(n) * sizeof(KeyT));
   L.valt = reinterpret_cast<uint32_t*>(take(size_t(n) * 4));
-  L.tot = reinterpret_cast<uint32_t*>(take(size_t(256) * 4));
+  L.tot = reinterpret_cast<uint32_t*>(take(size_t(256) * kSortMaxSegs * 4));
   L.cnt = reinterpret_cast<uint32_t*>(take(size_t(256) * col_stride(nb) * 4));
   L.total = o;
   return L;
@@ -216,7 +227,8 @@ SortLayout sort_layout(void* base, uint32_t n) {
 template <typename KeyT, int ITEMS>
 __global__ void __launch_bounds__(kSortBlock)
 radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ keys,
-                     uint32_t shift, uint32_t dbits, uint32_t nb, uint32_t* __restrict__ cnt) {
+                     uint32_t shift, uint32_t dbits, uint32_t nb, uint32_t cps,
+                     uint32_t* __restrict__ cnt) {
   constexpr int kTile = kSortBlock * ITEMS;
   __shared__ uint32_t h[256];
   const uint32_t n = load_n(n_cap, d_n);
@@ -225,7 +237,7 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   // capacity) have nothing to count: the column scan and the scatter stop
   // at the last tile holding an element
   if (blockIdx.x >= nbv) return;  // uniform
-  const uint32_t tb = xcd_tile(blockIdx.x, nbv);
+  const uint32_t tb = seg_tile(blockIdx.x, nbv, cps);
   const uint64_t base = uint64_t(tb) * kTile;
   const int lane = threadIdx.x & 63;
   h[threadIdx.x] = 0;
@@ -259,16 +271,20 @@ radix_upsweep_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     cnt[size_t(threadIdx.x) * col_stride(nb) + tb] = h[threadIdx.x];
 }
 
-// Workgroup d: cnt[d][b] <- elements of digit d in tiles before b;
-// tot[d] <- elements of digit d.
+// Workgroup (d, sg): cnt[d][b] <- elements of digit d in the tiles of
+// segment sg before b; tot[sg][d] <- elements of digit d in segment sg
+// (segments of cps tiles; cps = 0: one segment of all tiles).
 __global__ void __launch_bounds__(kSortBlock)
 radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb, uint32_t tile,
-                     uint32_t* __restrict__ cnt, uint32_t* __restrict__ tot) {
+                     uint32_t cps, uint32_t* __restrict__ cnt, uint32_t* __restrict__ tot) {
   const uint32_t n = load_n(n_cap, d_n);
-  const uint32_t nbv = div_up(n, tile);
-  const uint32_t d = blockIdx.x;
+  const uint32_t d = blockIdx.x, sg = blockIdx.y;
+  const uint32_t all = div_up(n, tile);
+  const uint32_t c0 = cps ? sg * cps : 0u;
+  const uint32_t nbv = cps ? min(cps, all > c0 ? all - c0 : 0u) : all;
   uint32_t carry = 0;
-  uint32_t* col = cnt + size_t(d) * col_stride(nb);
+  // the segment's column starts on a multiple of 8 words (cps is one)
+  uint32_t* col = cnt + size_t(d) * col_stride(nb) + c0;
   // column values per thread and round, as two 16-byte words (the column
   // starts on a multiple of 8 words and holds col_stride(nb) >= nbv of them)
   constexpr int kPer = 8, kSpan = kSortBlock * kPer;
@@ -304,7 +320,7 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb, uint32_t 
     }
     carry += chunk;
   }
-  if (threadIdx.x == 0) tot[d] = carry;
+  if (threadIdx.x == 0) tot[size_t(sg) * 256 + d] = carry;
 }
 
 // iota: the values are the element indices (vin is not read). cdst (may be
@@ -315,7 +331,8 @@ __global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
                      const uint32_t* __restrict__ vin, bool iota, KeyT* __restrict__ kout,
                      uint32_t* __restrict__ vout, uint32_t shift, uint32_t dbits,
-                     uint32_t nb, const uint32_t* __restrict__ off,
+                     uint32_t nb, uint32_t cps, uint32_t seg_len,
+                     const uint32_t* __restrict__ off,
                      const uint32_t* __restrict__ tot, uint32_t nbins,
                      const uint2* __restrict__ rsrc, uint32_t* __restrict__ cdst,
                      uint2* __restrict__ rdst) {
@@ -329,13 +346,14 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
   const uint32_t n = load_n(n_cap, d_n);
   const uint32_t nbv = div_up(n, uint32_t(kTile));
   if (blockIdx.x >= nbv) return;  // uniform
-  const uint32_t tb = xcd_tile(blockIdx.x, nbv);
+  const uint32_t tb = seg_tile(blockIdx.x, nbv, cps);
+  const uint32_t sg = cps ? tb / cps : 0u;
   const uint64_t bstart = uint64_t(tb) * kTile;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // the digit total and this tile's column prefix of digit threadIdx.x,
   // fetched now so their latency hides behind the ranking
   const bool live_digit = threadIdx.x < nbins;
-  const uint32_t my_tot = live_digit ? tot[threadIdx.x] : 0u;
+  const uint32_t my_tot = live_digit ? tot[size_t(sg) * 256 + threadIdx.x] : 0u;
   const uint32_t my_off =
       live_digit ? off[size_t(threadIdx.x) * col_stride(nb) + tb] : 0u;
   for (int k = threadIdx.x; k < kSortWaves * 256; k += kSortBlock) (&wcnt[0][0])[k] = 0;
@@ -379,13 +397,13 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     wcnt[k][threadIdx.x] = total;
     total += c;
   }
-  // dpre[d]: elements of the tile with a smaller digit; dbase[d]: elements
-  // of the whole array with a smaller digit, plus those of digit d in
-  // earlier tiles
+  // dpre[d]: elements of the tile with a smaller digit; dbase[d]: the
+  // segment's start, its elements with a smaller digit, and those of digit
+  // d in its earlier tiles
   uint32_t btot, all;
   dpre[threadIdx.x] = block_exclusive_scan(total, &btot);
   const uint32_t gpre = block_exclusive_scan(my_tot, &all);
-  if (live_digit) dbase[threadIdx.x] = gpre + my_off;
+  if (live_digit) dbase[threadIdx.x] = sg * seg_len + gpre + my_off;
   __syncthreads();
   // stage in LDS in tile-sorted order ...
 #pragma unroll
@@ -433,8 +451,11 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt_out, uint32_t** vals_alt_out,
                      bool iota_vals, const uint2* rect_src, uint32_t* cnt_dst,
-                     uint2* rect_dst) {
+                     uint2* rect_dst, uint32_t seg_len) {
   *result_in_alt = false;
+  if (seg_len && (d_n || seg_len % kSortSegAlign || n_cap % seg_len ||
+                  n_cap / seg_len > kSortMaxSegs))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "radix sort: bad segment length");
   if (ws_bytes < sort_workspace_bytes<KeyT>(n_cap))
     return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
   if (key_bits == 0 || key_bits > 8 * sizeof(KeyT))
@@ -447,6 +468,7 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
   const bool big = n_cap >= kBigSort;
   const uint32_t tile = kSortBlock * (big ? kSortItemsBig : kSortItems);
   const uint32_t nb = div_up(n_cap, tile);
+  const uint32_t cps = seg_len / tile, nseg = seg_len ? n_cap / seg_len : 1u;
   // as few passes as 8-bit digits allow, with the bits spread evenly over
   // them: fewer digits per pass give longer runs per digit in the scatter
   // (11 bits: 6 + 5 rather than 8 + 3)
@@ -460,11 +482,13 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
 #define DSPLAT_PASS(I_)                                                                        \
   do {                                                                                         \
     radix_upsweep_kernel<KeyT, I_><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift, dbits, nb, \
-                                                             L.cnt);                           \
-    radix_colscan_kernel<<<1u << dbits, kSortBlock, 0, s>>>(n_cap, d_n, nb, tile, L.cnt, L.tot); \
+                                                             cps, L.cnt);                      \
+    radix_colscan_kernel<<<dim3(1u << dbits, nseg), kSortBlock, 0, s>>>(n_cap, d_n, nb, tile,  \
+                                                                        cps, L.cnt, L.tot);    \
     radix_scatter_kernel<KeyT, I_><<<nb, kSortBlock, 0, s>>>(                                  \
-        n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, L.cnt,      \
-        L.tot, 1u << dbits, rect_src, last ? cnt_dst : nullptr, last ? rect_dst : nullptr);    \
+        n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, cps,        \
+        seg_len, L.cnt, L.tot, 1u << dbits, rect_src, last ? cnt_dst : nullptr,                \
+        last ? rect_dst : nullptr);                                                            \
   } while (0)
     if (big)
       DSPLAT_PASS(kSortItemsBig);
@@ -485,10 +509,12 @@ template size_t sort_workspace_bytes<uint32_t>(uint32_t);
 template size_t sort_workspace_bytes<uint64_t>(uint32_t);
 template int radix_sort_pairs<uint32_t>(hipStream_t, uint32_t, const uint32_t*, uint32_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint32_t**,
-                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*);
+                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*,
+                                        uint32_t);
 template int radix_sort_pairs<uint64_t>(hipStream_t, uint32_t, const uint32_t*, uint64_t*,
                                         uint32_t*, uint32_t, void*, size_t, bool*, uint64_t**,
-                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*);
+                                        uint32_t**, bool, const uint2*, uint32_t*, uint2*,
+                                        uint32_t);
 
 // ------------------------------------------------------------------ fill ----
 __global__ void fill_f32_kernel(uint32_t n, float* __restrict__ buf, float v) {

@@ -56,14 +56,22 @@ size_t sort_workspace_bytes(uint32_t n);
 // *result_in_alt is false, else in the workspace's alternate buffers
 // (*keys_alt, *vals_alt). iota_vals: the values are the indices 0..n-1 and
 // vals is not read (it still receives the result when *result_in_alt is
-// false). gather_dst (may be null): the last pass also writes
-// gather_dst[i] = gather_src[sorted value i].
+// false). rect_dst / cnt_dst (may be null): the last pass also writes
+// rect_dst[i] = rect_src[sorted value i] and cnt_dst[i] its area.
+// seg_len (0: one segment): the array is n_cap / seg_len segments of
+// seg_len elements, each sorted on its own and left in place (d_n null,
+// seg_len a multiple of kSortSegAlign, at most kSortMaxSegs segments).
+// Every pass then moves elements only within their segment, so the last
+// pass's gathers for a segment read the values one segment can hold.
+constexpr uint32_t kSortSegAlign = 32768;
+constexpr uint32_t kSortMaxSegs = 64;
 template <typename KeyT>
 int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* keys,
                      uint32_t* vals, uint32_t key_bits, void* ws, size_t ws_bytes,
                      bool* result_in_alt, KeyT** keys_alt, uint32_t** vals_alt,
                      bool iota_vals = false, const uint2* rect_src = nullptr,
-                     uint32_t* cnt_dst = nullptr, uint2* rect_dst = nullptr);
+                     uint32_t* cnt_dst = nullptr, uint2* rect_dst = nullptr,
+                     uint32_t seg_len = 0);
 
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 

@@ -63,6 +63,35 @@ def test_multiview_forward_matches_per_view(oracle, ts, threads):
     plan.close()
 
 
+@pytest.mark.parametrize("n", [1 << 18, 300000])
+def test_views_depth_sorted_in_segments_match_per_view(n):
+    """From 2^18 Gaussians the views are depth-sorted each in its own
+    segment, padded to whole segments (300000: 27680 padding entries per
+    view; 2^18: none). The tile lists and images must still equal the
+    single-view binning and raster per view, bit for bit."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack.SplatGauss.multiview import MultiViewPlan
+    s, views = _setup(n=n, seed=44, nview=3)
+    xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    ts = 16
+    plan = MultiViewPlan(n, views, tile_size=ts, threads=0)
+    out_rgb, _ = plan.forward(xyz, ls, q, op, rgb)
+    t2i, i2p = plan.lists()
+    g_t2i = u32(t2i).astype(np.int64)
+    g_i2p = u32(i2p).astype(np.int64)
+    imgs = plan.split(out_rgb, 3)
+    for v, (m, shape) in enumerate(views):
+        a, b = plan.tile_off[v], plan.tile_off[v + 1]
+        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, _t(m), shape)
+        s_t2i, s_i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+        assert g_t2i[b] - g_t2i[a] > 0
+        np.testing.assert_array_equal(g_t2i[a:b + 1] - g_t2i[a], u32(s_t2i))
+        np.testing.assert_array_equal(g_i2p[g_t2i[a]:g_t2i[b]] - v * n, u32(s_i2p))
+        img, _, _, _ = SG.rasterize_fwd(shape, ts, splat2, s_t2i, s_i2p, 0)
+        np.testing.assert_array_equal(imgs[v].cpu().numpy(), img.cpu().numpy())
+    plan.close()
+
+
 @pytest.mark.parametrize("ts,threads", [(16, 0), (32, 512)])
 def test_multiview_backward_matches_sum_of_per_view(ts, threads):
     import del_splat_dlpack.SplatGauss.torch as SG
