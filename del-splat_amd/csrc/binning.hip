@@ -79,14 +79,16 @@ sum_partials_kernel(uint32_t nb, const uint32_t* __restrict__ partial, uint32_t*
 // (tile_acceleration.h:29-50), value = Gaussian index. n_eff = K if K <=
 // capacity else 0 (overflow: nothing is written and the later phases see no
 // pairs); workgroup 0 stores it for them. The rectangles come in depth order
-// from the sort (read in sequence, no gather of boxes).
+// from the sort (read in sequence, no gather of boxes). pnt_of_pair null:
+// one packed word per pair, tile << ib | Gaussian index, in tile_of_pair.
 __global__ void __launch_bounds__(kScanBlock)
 bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
                 const uint2* __restrict__ rect_sorted, const uint32_t* __restrict__ cnt,
                 const uint32_t* __restrict__ chunk_prefix,
                 const uint32_t* __restrict__ d_k, uint32_t capacity,
                 uint32_t* __restrict__ n_eff_out, uint32_t tw,
-                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ pnt_of_pair) {
+                uint32_t* __restrict__ tile_of_pair, uint32_t* __restrict__ pnt_of_pair,
+                uint32_t ib) {
   __shared__ ExpandLds lds[4];
   const uint32_t k = *d_k;
   const uint32_t n_eff = k <= capacity ? k : 0u;
@@ -103,16 +105,19 @@ bin_fill_kernel(uint32_t n, const uint32_t* __restrict__ order,
   }
   uint32_t block_total;
   const uint32_t o = chunk_prefix[blockIdx.x] + block_exclusive_scan(c, &block_total);
-  wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair);
+  wave_expand_pairs(valid, o, r, 0u, tw, g, lds[threadIdx.x >> 6], tile_of_pair, pnt_of_pair,
+                    ib);
 }
 
 // tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T]
-// (tile_ranges4: four positions per thread).
+// (tile_ranges4: four positions per thread; the tile id of a sorted word is
+// word >> ib).
 __global__ void __launch_bounds__(256)
 tile_ranges_kernel(const uint32_t* __restrict__ n_eff, uint32_t capacity, uint32_t num_tile,
-                   const uint32_t* __restrict__ tile_sorted, uint32_t* __restrict__ tile2idx) {
+                   const uint32_t* __restrict__ tile_sorted, uint32_t ib,
+                   uint32_t* __restrict__ tile2idx) {
   const uint32_t k = min(*n_eff, capacity);
-  tile_ranges4(blockIdx.x * blockDim.x + threadIdx.x, k, num_tile, tile_sorted, tile2idx);
+  tile_ranges4(blockIdx.x * blockDim.x + threadIdx.x, k, num_tile, tile_sorted, tile2idx, ib);
 }
 
 __global__ void __launch_bounds__(256)
@@ -207,6 +212,9 @@ int bin_count(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth, u
   return DSPLAT_OK;
 }
 
+// Depth keys are the bits of a non-negative float: 31 significant bits.
+constexpr uint32_t kDepthBits = 31;
+
 // Phase 2: stable depth order of the Gaussians (values generated as iota by
 // the first pass; the last pass also gathers the tile rectangles and their
 // areas, the pair counts, into depth order),
@@ -220,7 +228,7 @@ inline int bin_order(hipStream_t s, uint32_t n, const PointLayout& L, uint32_t* 
   }
   bool in_alt = false;
   uint32_t *kalt = nullptr, *valt = nullptr;
-  int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, 31, L.dsort_ws,
+  int rc = radix_sort_pairs<uint32_t>(s, n, nullptr, L.dkey, L.order, kDepthBits, L.dsort_ws,
                                       L.dsort_bytes, &in_alt, &kalt, &valt, true, L.rect, L.off,
                                       L.rect_sorted);
   if (rc) return rc;
@@ -236,12 +244,37 @@ int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, u
              const PointLayout& L, const uint32_t* d_num_idx, uint32_t* tile2idx,
              uint32_t* idx2pnt, uint32_t capacity, const PairLayout& Q) {
   const uint32_t nt = tw * th;
+  const uint32_t tbits = bit_width(nt - 1), ib = bit_width(n > 1 ? n - 1 : 0);
+  if (capacity > 0 && nt > 1 && tbits + ib <= 32) {
+    // packed pairs, tile << ib | Gaussian index: the sort moves one word
+    // per pair and its last pass writes the indices into idx2pnt
+    if (n == 0) {
+      set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
+    } else {
+      bin_fill_kernel<<<div_up(n, kScanBlock), kScanBlock, 0, s>>>(
+          n, L.order, L.rect_sorted, L.off, L.scan_ws, d_num_idx, capacity, L.misc, tw,
+          Q.tilekey, nullptr, ib);
+    }
+    bool in_alt = false;
+    uint32_t* kalt = nullptr;
+    const int rc = radix_sort_keys(s, capacity, L.misc, Q.tilekey, ib, tbits,
+                                   ib ? (1u << ib) - 1u : 0u, idx2pnt, Q.tsort_ws, Q.tsort_bytes,
+                                   &in_alt, &kalt);
+    if (rc) return rc;
+    const uint32_t* ksorted = in_alt ? kalt : Q.tilekey;
+    if (reinterpret_cast<uintptr_t>(ksorted) & 15u)
+      return dsplat_set_error(DSPLAT_ERR_ARG, "binning: sorted pairs not 16-byte aligned");
+    tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 1024), 256, 0, s>>>(
+        L.misc, capacity, nt, ksorted, ib, tile2idx);
+    DSPLAT_HIP(hipGetLastError());
+    return DSPLAT_OK;
+  }
   if (n == 0) {
     set_u32_one<<<1, 1, 0, s>>>(L.misc, 0u);
   } else {
     bin_fill_kernel<<<div_up(n, kScanBlock), kScanBlock, 0, s>>>(
         n, L.order, L.rect_sorted, L.off, L.scan_ws, d_num_idx, capacity, L.misc, tw, Q.tilekey,
-        idx2pnt);
+        idx2pnt, 0u);
   }
   const uint32_t* tsorted = Q.tilekey;
   if (capacity > 0 && nt > 1) {
@@ -259,7 +292,7 @@ int bin_fill(hipStream_t s, uint32_t n, Boxes boxes, uint32_t ts, uint32_t tw, u
   if (reinterpret_cast<uintptr_t>(tsorted) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "binning: sorted tile ids not 16-byte aligned");
   tile_ranges_kernel<<<div_up(uint64_t(capacity) + 1, 1024), 256, 0, s>>>(L.misc, capacity, nt,
-                                                                          tsorted, tile2idx);
+                                                                          tsorted, 0u, tile2idx);
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
 }

@@ -96,23 +96,24 @@ __device__ inline uint32_t rect_area(uint2 p) {
 }
 
 // tile2idx[t] = first sorted position whose tile id is >= t, t in [0, T],
-// from k sorted tile ids (16-byte aligned), for the four positions
-// [4q, 4q + 4) of thread q (the position k stands for tile T): one 16-byte
-// load and the previous id, and a store for every tile a boundary passes.
+// from k sorted words (16-byte aligned) whose tile id is word >> ib, for the
+// four positions [4q, 4q + 4) of thread q (the position k stands for tile
+// T): one 16-byte load and the previous word, and a store for every tile a
+// boundary passes.
 __device__ inline void tile_ranges4(uint32_t q, uint32_t k, uint32_t num_tile,
                                     const uint32_t* __restrict__ tile_sorted,
-                                    uint32_t* __restrict__ tile2idx) {
+                                    uint32_t* __restrict__ tile2idx, uint32_t ib = 0) {
   const uint32_t i0 = 4u * q;
   if (i0 > k) return;
   uint32_t t[4];
   if (i0 + 4u <= k) {
     const uint4 v = reinterpret_cast<const uint4*>(tile_sorted)[q];
-    t[0] = v.x; t[1] = v.y; t[2] = v.z; t[3] = v.w;
+    t[0] = v.x >> ib; t[1] = v.y >> ib; t[2] = v.z >> ib; t[3] = v.w >> ib;
   } else {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) t[e] = i0 + e < k ? tile_sorted[i0 + e] : num_tile;
+    for (int e = 0; e < 4; ++e) t[e] = i0 + e < k ? tile_sorted[i0 + e] >> ib : num_tile;
   }
-  uint32_t lo = i0 == 0 ? 0u : tile_sorted[i0 - 1] + 1u;
+  uint32_t lo = i0 == 0 ? 0u : (tile_sorted[i0 - 1] >> ib) + 1u;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const uint32_t idx = i0 + uint32_t(e);
@@ -167,11 +168,12 @@ __device__ inline uint32_t wave_incl_sum_u32(uint32_t v) {
 // range in chunks of 64 (every lane one slot, so the stores are coalesced);
 // the item of a slot is the last one starting at or before it: each item
 // with a non-empty range marks its first slot, and a prefix max over the
-// chunk (carried across chunks) spreads the marks.
+// chunk (carried across chunks) spreads the marks. val_out null: one packed
+// word per pair, tile << ib | value, into tile_out (values below 2^ib).
 __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, uint32_t tbase,
                                          uint32_t tw, uint32_t val, ExpandLds& L,
                                          uint32_t* __restrict__ tile_out,
-                                         uint32_t* __restrict__ val_out) {
+                                         uint32_t* __restrict__ val_out, uint32_t ib = 0) {
   const int lane = threadIdx.x & 63;
   const uint64_t vmask = __ballot(valid);
   if (vmask == 0) return;
@@ -209,8 +211,13 @@ __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, u
       while (qx * h > c) --qx;
       while ((qx + 1u) * h <= c) ++qx;
       const uint32_t ix = L.x0[lo] + qx, iy = L.y0[lo] + (c - qx * h);
-      tile_out[e] = L.tbase[lo] + iy * L.tw[lo] + ix;
-      val_out[e] = L.val[lo];
+      const uint32_t tile = L.tbase[lo] + iy * L.tw[lo] + ix;
+      if (val_out) {
+        tile_out[e] = tile;
+        val_out[e] = L.val[lo];
+      } else {
+        tile_out[e] = (tile << ib) | L.val[lo];
+      }
     }
   }
   __builtin_amdgcn_wave_barrier();

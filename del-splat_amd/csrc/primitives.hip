@@ -163,13 +163,19 @@ int scan_chunk_prefixes(hipStream_t s, uint32_t n, const uint32_t* in, uint32_t*
 // the digit's total), scatter (digit bases from the totals, then the stable
 // in-tile rank).
 constexpr int kSortBlock = 256;
-// Rounds per wave: 8 (2048-element tiles), or 16 (4096) for sorts of at
-// least kBigSort elements, whose scatter then writes runs twice as long per
-// digit (fewer partial lines) over tiles that still fill the GPU.
+// Rounds per wave: 8 (2048-element tiles); 16 (4096) for sorts of at least
+// kBigSort elements, whose scatter then writes runs twice as long per digit
+// (fewer partial lines) over tiles that still fill the GPU; 4 (1024) for
+// sorts below kMidSort elements, which are latency-bound: more, shorter
+// workgroups (the 1M-element depth sort per pass 23 -> 20 us).
+constexpr int kSortItemsSmall = 4;
 constexpr int kSortItems = 8;
 constexpr int kSortItemsBig = 16;
 constexpr uint32_t kBigSort = 8u << 20;
-constexpr int kSortTile = kSortBlock * kSortItems;   // the smaller tile (sizes the workspace)
+constexpr uint32_t kMidSort = 2u << 20;
+inline int sort_items(uint32_t n) {
+  return n >= kBigSort ? kSortItemsBig : n >= kMidSort ? kSortItems : kSortItemsSmall;
+}
 constexpr int kSortWaves = kSortBlock / 64;
 
 struct SortLayout {
@@ -207,7 +213,7 @@ __device__ inline uint32_t seg_tile(uint32_t b, uint32_t nbv, uint32_t cps) {
 
 template <typename KeyT>
 SortLayout sort_layout(void* base, uint32_t n) {
-  const uint32_t nb = div_up(n ? n : 1, kSortTile);
+  const uint32_t nb = div_up(n ? n : 1, kSortBlock * sort_items(n));
   SortLayout L{};
   char* p = static_cast<char*>(base);
   size_t o = 0;
@@ -325,8 +331,10 @@ radix_colscan_kernel(uint32_t n_cap, const uint32_t* d_n, uint32_t nb, uint32_t 
 
 // iota: the values are the element indices (vin is not read). cdst (may be
 // null): also gathers rsrc[value], a packed tile rectangle, writing it to
-// rdst[pos] and its area to cdst[pos].
-template <typename KeyT, int ITEMS>
+// rdst[pos] and its area to cdst[pos]. Without values (VALS false) the keys
+// alone move: kout (may be null) receives them and vout (may be null) the
+// key's low bits, key & vmask.
+template <typename KeyT, int ITEMS, bool VALS>
 __global__ void __launch_bounds__(kSortBlock)
 radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict__ kin,
                      const uint32_t* __restrict__ vin, bool iota, KeyT* __restrict__ kout,
@@ -335,14 +343,14 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
                      const uint32_t* __restrict__ off,
                      const uint32_t* __restrict__ tot, uint32_t nbins,
                      const uint2* __restrict__ rsrc, uint32_t* __restrict__ cdst,
-                     uint2* __restrict__ rdst) {
+                     uint2* __restrict__ rdst, uint32_t vmask) {
   static_assert(kSortBlock == 256, "one thread per digit");
   constexpr int kTile = kSortBlock * ITEMS, kWaveChunk = 64 * ITEMS;
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dpre[256];
   __shared__ uint32_t dbase[256];
   __shared__ KeyT skey[kTile];
-  __shared__ uint32_t sval[kTile];
+  __shared__ uint32_t sval[VALS ? kTile : 1];
   const uint32_t n = load_n(n_cap, d_n);
   const uint32_t nbv = div_up(n, uint32_t(kTile));
   if (blockIdx.x >= nbv) return;  // uniform
@@ -367,7 +375,7 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     const uint64_t i = base + uint64_t(r) * 64 + lane;
     const bool valid = i < n;
     key[r] = valid ? kin[i] : KeyT(0);
-    val[r] = valid ? (iota ? uint32_t(i) : vin[i]) : 0u;
+    if (VALS) val[r] = valid ? (iota ? uint32_t(i) : vin[i]) : 0u;
   }
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
@@ -412,12 +420,26 @@ radix_scatter_kernel(uint32_t n_cap, const uint32_t* d_n, const KeyT* __restrict
     const uint32_t d = dig[r];
     const uint32_t lp = dpre[d] + wcnt[w][d] + rank[r];
     skey[lp] = key[r];
-    sval[lp] = val[r];
+    if (VALS) sval[lp] = val[r];
   }
   __syncthreads();
   // ... and write each digit's run to consecutive global addresses (the
   // gathered values loaded for all rounds before any is stored)
   const uint32_t nvalid = uint32_t(min(uint64_t(kTile), n - bstart));
+  if (!VALS) {
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+      const uint32_t i = uint32_t(r) * kSortBlock + threadIdx.x;
+      if (i < nvalid) {
+        const KeyT k = skey[i];
+        const uint32_t d = uint32_t(k >> shift) & mask;
+        const uint32_t pos = dbase[d] + (i - dpre[d]);
+        if (kout) kout[pos] = k;
+        if (vout) vout[pos] = uint32_t(k) & vmask;
+      }
+    }
+    return;
+  }
   uint2 gv[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
@@ -465,8 +487,8 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
   if (keys_alt_out) *keys_alt_out = kalt;
   if (vals_alt_out) *vals_alt_out = L.valt;
   if (n_cap == 0) return DSPLAT_OK;
-  const bool big = n_cap >= kBigSort;
-  const uint32_t tile = kSortBlock * (big ? kSortItemsBig : kSortItems);
+  const int items = sort_items(n_cap);
+  const uint32_t tile = kSortBlock * items;
   const uint32_t nb = div_up(n_cap, tile);
   const uint32_t cps = seg_len / tile, nseg = seg_len ? n_cap / seg_len : 1u;
   // as few passes as 8-bit digits allow, with the bits spread evenly over
@@ -485,20 +507,72 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
                                                              cps, L.cnt);                      \
     radix_colscan_kernel<<<dim3(1u << dbits, nseg), kSortBlock, 0, s>>>(n_cap, d_n, nb, tile,  \
                                                                         cps, L.cnt, L.tot);    \
-    radix_scatter_kernel<KeyT, I_><<<nb, kSortBlock, 0, s>>>(                                  \
+    radix_scatter_kernel<KeyT, I_, true><<<nb, kSortBlock, 0, s>>>(                            \
         n_cap, d_n, ksrc, vsrc, iota_vals && p == 0, kdst, vdst, shift, dbits, nb, cps,        \
         seg_len, L.cnt, L.tot, 1u << dbits, rect_src, last ? cnt_dst : nullptr,                \
-        last ? rect_dst : nullptr);                                                            \
+        last ? rect_dst : nullptr, 0u);                                                        \
   } while (0)
-    if (big)
-      DSPLAT_PASS(kSortItemsBig);
+    if (items == 16)
+      DSPLAT_PASS(16);
+    else if (items == 8)
+      DSPLAT_PASS(8);
     else
-      DSPLAT_PASS(kSortItems);
+      DSPLAT_PASS(4);
 #undef DSPLAT_PASS
     std::swap(ksrc, kdst);
     std::swap(vsrc, vdst);
     in_alt = !in_alt;
     shift += dbits;
+  }
+  DSPLAT_HIP(hipGetLastError());
+  *result_in_alt = in_alt;
+  return DSPLAT_OK;
+}
+
+int radix_sort_keys(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, uint32_t* keys,
+                    uint32_t shift0, uint32_t key_bits, uint32_t vmask, uint32_t* vout, void* ws,
+                    size_t ws_bytes, bool* result_in_alt, uint32_t** keys_alt_out) {
+  *result_in_alt = false;
+  if (ws_bytes < sort_workspace_bytes<uint32_t>(n_cap))
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "radix sort: workspace too small");
+  if (key_bits == 0 || shift0 + key_bits > 32)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "radix sort: key bits out of range");
+  const SortLayout L = sort_layout<uint32_t>(ws, n_cap);
+  uint32_t* kalt = static_cast<uint32_t*>(L.kalt);
+  if (keys_alt_out) *keys_alt_out = kalt;
+  if (n_cap == 0) return DSPLAT_OK;
+  const int items = sort_items(n_cap);
+  const uint32_t tile = kSortBlock * items;
+  const uint32_t nb = div_up(n_cap, tile);
+  const uint32_t P = div_up(key_bits, 8u);
+  uint32_t* ksrc = keys;
+  uint32_t* kdst = kalt;
+  bool in_alt = false;
+  uint32_t used = 0;
+  for (uint32_t p = 0; p < P; ++p) {
+    const uint32_t dbits = div_up(key_bits - used, P - p);
+    const uint32_t shift = shift0 + used;
+    uint32_t* vo = p + 1 == P ? vout : nullptr;
+#define DSPLAT_KPASS(I_)                                                                        \
+  do {                                                                                          \
+    radix_upsweep_kernel<uint32_t, I_><<<nb, kSortBlock, 0, s>>>(n_cap, d_n, ksrc, shift,       \
+                                                                 dbits, nb, 0u, L.cnt);         \
+    radix_colscan_kernel<<<dim3(1u << dbits, 1), kSortBlock, 0, s>>>(n_cap, d_n, nb, tile, 0u,  \
+                                                                     L.cnt, L.tot);             \
+    radix_scatter_kernel<uint32_t, I_, false><<<nb, kSortBlock, 0, s>>>(                        \
+        n_cap, d_n, ksrc, nullptr, false, kdst, vo, shift, dbits, nb, 0u, 0u, L.cnt, L.tot,     \
+        1u << dbits, nullptr, nullptr, nullptr, vmask);                                         \
+  } while (0)
+    if (items == 16)
+      DSPLAT_KPASS(16);
+    else if (items == 8)
+      DSPLAT_KPASS(8);
+    else
+      DSPLAT_KPASS(4);
+#undef DSPLAT_KPASS
+    std::swap(ksrc, kdst);
+    in_alt = !in_alt;
+    used += dbits;
   }
   DSPLAT_HIP(hipGetLastError());
   *result_in_alt = in_alt;

@@ -73,6 +73,15 @@ int radix_sort_pairs(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, KeyT* k
                      uint32_t* cnt_dst = nullptr, uint2* rect_dst = nullptr,
                      uint32_t seg_len = 0);
 
+// Stable LSD radix sort of n = min(*d_n, n_cap) u32 keys by their bits
+// [shift0, shift0 + key_bits) alone (packed keys: no values move). The last
+// pass also writes vout[i] = (sorted key i) & vmask (vout may be null). The
+// sorted keys are in keys if *result_in_alt is false, else in *keys_alt.
+// ws: sort_workspace_bytes<uint32_t>(n_cap) bytes.
+int radix_sort_keys(hipStream_t s, uint32_t n_cap, const uint32_t* d_n, uint32_t* keys,
+                    uint32_t shift0, uint32_t key_bits, uint32_t vmask, uint32_t* vout, void* ws,
+                    size_t ws_bytes, bool* result_in_alt, uint32_t** keys_alt);
+
 int fill_f32(hipStream_t s, uint32_t n, float* buf, float v);
 
 // order[0..n) = 0..n-1 by descending cost (ties in any order): a counting
