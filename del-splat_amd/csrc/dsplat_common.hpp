@@ -131,10 +131,12 @@ __device__ inline void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-wave LDS slice of wave_expand_pairs.
+// Per-wave LDS slice of wave_expand_pairs. An item's fields are packed so
+// that a slot reads its item with one 16-byte and one 8-byte LDS load.
 struct ExpandLds {
-  uint32_t own[64];  // per slot of the current chunk: 1 + lane whose range starts there
-  uint32_t off[64], x0[64], y0[64], h[64], tbase[64], tw[64], val[64];
+  uint32_t own[128];  // per slot of the current chunk: 1 + lane whose range starts there
+  uint4 rec[64];     // off, x0 | y0 << 16, rectangle height, value
+  uint2 tt[64];      // tile id base, tile grid width
 };
 
 // Inclusive prefix max over the 64 lanes of a wave (row shifts by 1, 2, 4,
@@ -165,11 +167,12 @@ __device__ inline uint32_t wave_incl_sum_u32(uint32_t v) {
 // item l owns output slots [off, off + count) and covers the tile rectangle
 // r (ix-outer, iy-inner, tile id = tbase + iy * tw + ix). Items are valid on
 // a prefix of the lanes with non-decreasing offsets. The wave walks its slot
-// range in chunks of 64 (every lane one slot, so the stores are coalesced);
-// the item of a slot is the last one starting at or before it: each item
-// with a non-empty range marks its first slot, and a prefix max over the
-// chunk (carried across chunks) spreads the marks. val_out null: one packed
-// word per pair, tile << ib | value, into tile_out (values below 2^ib).
+// range in chunks of 128 (lane l takes slots 2l and 2l + 1, so the stores
+// cover the chunk contiguously); the item of a slot is the last one starting
+// at or before it: each item with a non-empty range marks its first slot,
+// and a prefix max over the chunk (carried across chunks) spreads the marks.
+// val_out null: one packed word per pair, tile << ib | value, into tile_out
+// (values below 2^ib).
 __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, uint32_t tbase,
                                          uint32_t tw, uint32_t val, ExpandLds& L,
                                          uint32_t* __restrict__ tile_out,
@@ -179,44 +182,48 @@ __device__ inline void wave_expand_pairs(bool valid, uint32_t off, TileRect r, u
   if (vmask == 0) return;
   const int nvalid = __popcll(vmask);
   const uint32_t cnt = valid ? uint32_t(r.count()) : 0u;
-  L.off[lane] = off;
-  L.x0[lane] = uint32_t(r.x0);
-  L.y0[lane] = uint32_t(r.y0);
-  L.h[lane] = uint32_t(r.y1 - r.y0);
-  L.tbase[lane] = tbase;
-  L.tw[lane] = tw;
-  L.val[lane] = val;
+  // tile coordinates are below 65536 (the callers check the grid)
+  L.rec[lane] = make_uint4(off, uint32_t(r.x0) | (uint32_t(r.y0) << 16), uint32_t(r.y1 - r.y0),
+                           val);
+  L.tt[lane] = make_uint2(tbase, tw);
   const uint32_t base = __builtin_amdgcn_readfirstlane(off);
   const uint32_t end = __builtin_amdgcn_readlane(off + cnt, nvalid - 1);
   uint32_t carry = 0;  // 1 + item owning the slot before the chunk
-  for (uint32_t c0 = base; c0 < end; c0 += 64) {
-    L.own[lane] = 0;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (cnt > 0 && off >= c0 && off - c0 < 64u) L.own[off - c0] = uint32_t(lane) + 1u;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t own = wave_incl_max_u32(max(L.own[lane], lane == 0 ? carry : 0u));
-    carry = __builtin_amdgcn_readlane(own, 63);
-    const uint32_t e = c0 + uint32_t(lane);
-    if (e < end) {
-      const uint32_t lo = own - 1u;
-      const uint32_t c = e - L.off[lo];
-      const uint32_t h = L.h[lo];
-      // c / h via the float reciprocal, then corrected to the exact
-      // quotient (off by at most one for c < 2^22; the loops make it exact)
-      uint32_t qx = uint32_t(float(c) * __builtin_amdgcn_rcpf(float(h)));
-      while (qx * h > c) --qx;
-      while ((qx + 1u) * h <= c) ++qx;
-      const uint32_t ix = L.x0[lo] + qx, iy = L.y0[lo] + (c - qx * h);
-      const uint32_t tile = L.tbase[lo] + iy * L.tw[lo] + ix;
-      if (val_out) {
-        tile_out[e] = tile;
-        val_out[e] = L.val[lo];
-      } else {
-        tile_out[e] = (tile << ib) | L.val[lo];
+  for (uint32_t c0 = base; c0 < end; c0 += 128) {
+    reinterpret_cast<uint2*>(L.own)[lane] = make_uint2(0u, 0u);
+    wave_lds_fence();
+    if (cnt > 0 && off >= c0 && off - c0 < 128u) L.own[off - c0] = uint32_t(lane) + 1u;
+    wave_lds_fence();
+    const uint2 o = reinterpret_cast<const uint2*>(L.own)[lane];
+    const uint32_t o0 = lane == 0 ? max(o.x, carry) : o.x;
+    // owner of slot 2l + 1: the prefix max up to lane l; of slot 2l: the
+    // prefix max up to lane l - 1, with slot 2l's own mark
+    const uint32_t incl = wave_incl_max_u32(max(o0, o.y));
+    const uint32_t prev = uint32_t(__shfl_up(int(incl), 1, 64));
+    const uint32_t own[2] = {max(lane == 0 ? 0u : prev, o0), incl};
+    carry = __builtin_amdgcn_readlane(incl, 63);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t e = c0 + 2u * uint32_t(lane) + uint32_t(k);
+      if (e < end) {
+        const uint32_t lo = own[k] - 1u;
+        const uint4 q = L.rec[lo];
+        const uint2 t2 = L.tt[lo];
+        const uint32_t c = e - q.x;
+        const uint32_t h = q.z;
+        // c / h via the float reciprocal, then corrected to the exact
+        // quotient (off by at most one for c < 2^22; the loops make it exact)
+        uint32_t qx = uint32_t(float(c) * __builtin_amdgcn_rcpf(float(h)));
+        while (qx * h > c) --qx;
+        while ((qx + 1u) * h <= c) ++qx;
+        const uint32_t ix = (q.y & 0xFFFFu) + qx, iy = (q.y >> 16) + (c - qx * h);
+        const uint32_t tile = t2.x + iy * t2.y + ix;
+        if (val_out) {
+          tile_out[e] = tile;
+          val_out[e] = q.w;
+        } else {
+          tile_out[e] = (tile << ib) | q.w;
+        }
       }
     }
   }
