@@ -159,15 +159,14 @@ __device__ inline bool project_bwd_one(const float* __restrict__ m, uint32_t img
     for (int k = 0; k < 3; ++k)
       g[j + 3 * k] = p.wpix[2 * j] * d_wr[2 * k] + p.wpix[1 + 2 * j] * d_wr[1 + 2 * k];
   const float x = qt.x, y = qt.y, z = qt.z, w = qt.w;
-  // dR/d(x,y,z,w), column-major entries
-  dq[0] = g[1] * 2.f * y + g[2] * 2.f * z + g[3] * 2.f * y - g[4] * 4.f * x +
-          g[5] * 2.f * w + g[6] * 2.f * z - g[7] * 2.f * w - g[8] * 4.f * x;
-  dq[1] = -g[0] * 4.f * y + g[1] * 2.f * x - g[2] * 2.f * w + g[3] * 2.f * x +
-          g[5] * 2.f * z + g[6] * 2.f * w + g[7] * 2.f * z - g[8] * 4.f * y;
-  dq[2] = -g[0] * 4.f * z + g[1] * 2.f * w + g[2] * 2.f * x - g[3] * 2.f * w -
-          g[4] * 4.f * z + g[5] * 2.f * y + g[6] * 2.f * x + g[7] * 2.f * y;
-  dq[3] = g[1] * 2.f * z - g[2] * 2.f * y - g[3] * 2.f * z + g[5] * 2.f * x +
-          g[6] * 2.f * y - g[7] * 2.f * x;
+  // dR/d(x,y,z,w), column-major entries; the symmetric and antisymmetric
+  // off-diagonal pairs of g enter together (the factors 2 and 4 are exact)
+  const float p13 = g[1] + g[3], p26 = g[2] + g[6], p57 = g[5] + g[7];
+  const float m13 = g[1] - g[3], m62 = g[6] - g[2], m57 = g[5] - g[7];
+  dq[0] = 2.f * (p13 * y + p26 * z + m57 * w) - 4.f * x * (g[4] + g[8]);
+  dq[1] = 2.f * (p13 * x + m62 * w + p57 * z) - 4.f * y * (g[0] + g[8]);
+  dq[2] = 2.f * (p26 * x + p57 * y + m13 * w) - 4.f * z * (g[0] + g[4]);
+  dq[3] = 2.f * (m13 * z + m62 * y + m57 * x);
   // Wpix = A J -> dL/dJ = A^T dWpix, A = [a0 0 a4; 0 a3 a5]
   const float a0 = 0.5f * float(img_w), a3 = -0.5f * float(img_h);
   const float a4 = 0.5f * float(img_w), a5 = 0.5f * float(img_h);
@@ -180,16 +179,21 @@ __device__ inline bool project_bwd_one(const float* __restrict__ m, uint32_t img
   }
   const float d_q0 = a0 * gp.x, d_q1 = a3 * gp.y;
   const float inv_w = 1.f / p.w;
+  // dJ_rj/dx_k = -(J_rk M_3j + J_rj M_3k) / w, so
+  // sum_rj dL/dJ_rj dJ_rj/dx_k = -(sum_r J_rk u_r + M_3k t) / w with
+  // u_r = sum_j dL/dJ_rj M_3j and t = sum_rj dL/dJ_rj J_rj
+  float u[3], t = 0.f;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    u[r] = d_jac[r] * m[3] + d_jac[r + 3] * m[7] + d_jac[r + 6] * m[11];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) t += d_jac[r + 3 * j] * p.jac[r + 3 * j];
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float s = d_q0 * p.jac[3 * k] + d_q1 * p.jac[1 + 3 * k];
-    // dJ_rj/dx_k = -(J_rk M_3j + J_rj M_3k) / w
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-        s -= d_jac[r + 3 * j] * (p.jac[r + 3 * k] * m[3 + 4 * j] + p.jac[r + 3 * j] * m[3 + 4 * k]) * inv_w;
-    dx[k] = s;
+    const float sj = p.jac[3 * k] * u[0] + p.jac[1 + 3 * k] * u[1] + p.jac[2 + 3 * k] * u[2] +
+                     m[3 + 4 * k] * t;
+    dx[k] = d_q0 * p.jac[3 * k] + d_q1 * p.jac[1 + 3 * k] - sj * inv_w;
   }
   return true;
 }
