@@ -28,6 +28,11 @@ struct ProjState {
   bool culled;
 };
 
+// kFast: the divisions by w and by the determinant become one reciprocal
+// each and products (the adjoint's recomputation, whose bar is a relative
+// tolerance); otherwise every operation is rounded as written, the forward's
+// bit-exact order.
+template <bool kFast = false>
 __device__ inline void project_state(const float* __restrict__ m, float x0, float x1, float x2,
                                      float qx, float qy, float qz, float qw, float s0, float s1,
                                      float s2, uint32_t img_w, uint32_t img_h, ProjState& p) {
@@ -36,13 +41,24 @@ __device__ inline void project_state(const float* __restrict__ m, float x0, floa
 #pragma unroll
   for (int i = 0; i < 4; ++i) y[i] = m[i] * x0 + m[i + 4] * x1 + m[i + 8] * x2 + m[i + 12];
   p.w = y[3];
-  p.q[0] = y[0] / y[3];
-  p.q[1] = y[1] / y[3];
-  p.q[2] = y[2] / y[3];
+  if (kFast) {
+    const float iw = 1.f / y[3];
+    p.q[0] = y[0] * iw;
+    p.q[1] = y[1] * iw;
+    p.q[2] = y[2] * iw;
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < 3; ++j)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) p.jac[i + 3 * j] = (m[i + 4 * j] - p.q[i] * m[3 + 4 * j]) / p.w;
+      for (int i = 0; i < 3; ++i) p.jac[i + 3 * j] = (m[i + 4 * j] - p.q[i] * m[3 + 4 * j]) * iw;
+  } else {
+    p.q[0] = y[0] / y[3];
+    p.q[1] = y[1] / y[3];
+    p.q[2] = y[2] / y[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) p.jac[i + 3 * j] = (m[i + 4 * j] - p.q[i] * m[3 + 4 * j]) / p.w;
+  }
   const float a0 = 0.5f * float(img_w), a1 = 0.f, a2 = 0.f, a3 = -0.5f * float(img_h);
   const float a4 = 0.5f * float(img_w), a5 = 0.5f * float(img_h);
   p.pix[0] = a0 * p.q[0] + a2 * p.q[1] + a4;
@@ -81,9 +97,16 @@ __device__ inline void project_state(const float* __restrict__ m, float x0, floa
     const float r = sqrtf(h * h + b * b);
     if ((a + c) * 0.5f - r >= 1.0e-5f) {
       const float det = a * c - b * b;
-      p.conic[0] = c / det;
-      p.conic[1] = -b / det;
-      p.conic[2] = a / det;
+      if (kFast) {
+        const float idet = 1.f / det;
+        p.conic[0] = c * idet;
+        p.conic[1] = -b * idet;
+        p.conic[2] = a * idet;
+      } else {
+        p.conic[0] = c / det;
+        p.conic[1] = -b / det;
+        p.conic[2] = a / det;
+      }
     } else {
       float l0 = (a + c) * 0.5f + r;
       float l1 = (a + c) * 0.5f - r;
@@ -125,8 +148,8 @@ __device__ inline bool project_bwd_one(const float* __restrict__ m, uint32_t img
                                        const float gcon[3], float dx[3], float dsc[3],
                                        float dq[4]) {
   ProjState p;
-  project_state(m, x3[0], x3[1], x3[2], qt.x, qt.y, qt.z, qt.w, sg[0], sg[1], sg[2], img_w,
-                img_h, p);
+  project_state<true>(m, x3[0], x3[1], x3[2], qt.x, qt.y, qt.z, qt.w, sg[0], sg[1], sg[2], img_w,
+                      img_h, p);
   if (p.culled) return false;
   // d cov = -C G C with G the symmetric matrix of dL/dconic (off-diagonal halved)
   const float ca = p.conic[0], cb = p.conic[1], cc = p.conic[2];
