@@ -389,6 +389,7 @@ def headline(args, torch, np, SG, scene):
     evs = [pairs(["raster_bwd"]) for _ in range(args.steps)]
     n_stage = max(3, min(args.steps, 10))
     evs_stage = [pairs(stages) for _ in range(n_stage)]
+    evs_fused = [pairs(["project_bin"]) for _ in range(n_stage)]
     # binning without a host wait: the pair capacity comes from a K-read-back
     # binning of the same frame before the timed region (with slack); every
     # timed frame's K stays on the device and is checked after the last step
@@ -405,20 +406,37 @@ def headline(args, torch, np, SG, scene):
         t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, capacity[0], d_nums[i])
         return t2i, i2p
 
-    def step(ev, i=-1):
+    # the binning workspace of the one-call projection + binning, reused
+    bin_ws = [None]
+    if not args.sync_k:
+        from del_splat_dlpack import _native
+        bin_ws[0] = torch.empty(_native.lib().dsplat_tile2idx_idx2pnt_workspace_size(
+            args.num_gauss, tw * th, capacity[0]), device=dev, dtype=torch.uint8)
+
+    def step(ev, i=-1, split=False):
         def mark(name, first):
             if ev is not None and name in ev:
                 ev[name][0 if first else 1].record()
-        mark("project", True)
-        # the projection writes the records, depth and boxes, and clears the
-        # adjoint's accumulators
         gacc = torch.empty(9 * args.num_gauss, device=dev, dtype=torch.float32)
-        pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape, with_conic=False,
-                                              grad_acc=gacc)
-        mark("project", False)
-        mark("bin", True)
-        t2i, i2p = bin_frame(pxd, aabb, i)
-        mark("bin", False)
+        if args.sync_k or split:
+            mark("project", True)
+            # the projection writes the records, depth and boxes, and clears
+            # the adjoint's accumulators
+            pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape,
+                                                  with_conic=False, grad_acc=gacc)
+            mark("project", False)
+            mark("bin", True)
+            t2i, i2p = bin_frame(pxd, aabb, i)
+            mark("bin", False)
+        else:
+            # one call: the projection kernel also computes the binning's
+            # per-Gaussian tile rectangles and depth keys (no count kernel, no
+            # pixel/depth and box arrays), then the binning without a host wait
+            mark("project_bin", True)
+            splat2, t2i, i2p, _, _, _ = SG.project_bin(
+                xyz, ls, q, op, rgb, m16, shape, ts, capacity[0], grad_acc=gacc,
+                d_num_idx=d_nums[i], workspace=bin_ws[0])
+            mark("project_bin", False)
         mark("raster_fwd", True)
         cost = torch.empty(tw * th, device=dev, dtype=torch.int32)
         img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads,
@@ -452,16 +470,25 @@ def headline(args, torch, np, SG, scene):
             raise SystemExit(f"bench: pair capacity {capacity[0]} vs K {ks}: frames not binned")
         K = ks[-1]
     bwd_ms = sum(ev["raster_bwd"][0].elapsed_time(ev["raster_bwd"][1]) for ev in evs) / args.steps
+    # the stages of the projection and the binning apart (two calls), then
+    # the one call the timed steps make
     for i in range(n_stage):
-        step(evs_stage[i], args.steps)
+        step(evs_stage[i], args.steps, split=True)
+    if not args.sync_k:
+        for i in range(n_stage):
+            step(evs_fused[i], args.steps)
     torch.cuda.synchronize()
     stage_ms = {k: sum(ev[k][0].elapsed_time(ev[k][1]) for ev in evs_stage) / n_stage
                 for k in stages}
+    if not args.sync_k:
+        stage_ms["project_bin"] = sum(ev["project_bin"][0].elapsed_time(ev["project_bin"][1])
+                                      for ev in evs_fused) / n_stage
     stage_ms["raster_bwd_timed"] = bwd_ms
     # evaluations of the last frame: the sum of the per-pixel walk lengths
     evals = int(nwalk.to(torch.int64).sum().item())
     args.binning = ("K read back per frame" if args.sync_k else
-                    f"no host sync, pair capacity {capacity[0]}")
+                    f"no host sync, pair capacity {capacity[0]}, projection and binning in "
+                    "one call (dsplat_gauss_project_bin_ws)")
     return s, m, shape, ms, stage_ms, K, evals
 
 

@@ -15,6 +15,7 @@
 // first sort. Tile ranges are then the boundaries of the sorted tile ids.
 #include <mutex>
 
+#include "binning.hpp"
 #include "dsplat_common.hpp"
 #include "primitives.hpp"
 
@@ -310,6 +311,30 @@ int bin_and_sort(hipStream_t s, uint32_t n, Boxes boxes, const float* pixxydepth
   int rc = bin_count(s, n, boxes, pixxydepth, ts, tw, th, L, nullptr);
   if (!rc) rc = bin_order(s, n, L, d_num_idx);
   if (!rc) rc = bin_fill(s, n, boxes, ts, tw, th, L, d_num_idx, tile2idx, idx2pnt, capacity, Q);
+  return rc;
+}
+
+size_t bin_workspace_bytes(uint32_t n, uint32_t capacity) {
+  return point_layout(nullptr, n).total + pair_layout(nullptr, capacity).total;
+}
+
+void bin_count_slots(void* ws, uint32_t n, uint2** rect, uint32_t** dkey) {
+  const PointLayout L = point_layout(ws, n);
+  *rect = L.rect;
+  *dkey = L.dkey;
+}
+
+int bin_after_count(hipStream_t s, uint32_t n, uint32_t ts, uint32_t tw, uint32_t th,
+                    uint32_t* tile2idx, uint32_t* idx2pnt, uint32_t capacity, void* ws,
+                    size_t ws_bytes, uint32_t* d_num_idx) {
+  const PointLayout L = point_layout(ws, n);
+  const PairLayout Q = pair_layout(static_cast<char*>(ws) + L.total, capacity);
+  if (ws_bytes < L.total + Q.total)
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "binning: workspace too small");
+  int rc = bin_order(s, n, L, d_num_idx);
+  if (!rc)
+    rc = bin_fill(s, n, GaussBoxes{nullptr}, ts, tw, th, L, d_num_idx, tile2idx, idx2pnt,
+                  capacity, Q);
   return rc;
 }
 

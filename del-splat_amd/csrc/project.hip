@@ -1,8 +1,18 @@
 // Gaussian 3D->2D projection and its adjoint (single view). The math lives
 // in project_math.hpp (semantics of splat_gauss.cu:10-53).
+#include "binning.hpp"
 #include "project_math.hpp"
 
 namespace dsplat {
+
+// Binning outputs of the projection (BinOut.rect null: none): each
+// Gaussian's tile rectangle and depth key, as the binning's count phase
+// derives them from the box and the depth.
+struct BinOut {
+  uint2* rect;
+  uint32_t* dkey;
+  uint32_t ts, tw, th;
+};
 
 template <bool kLogScale, bool kPacked>
 __global__ void __launch_bounds__(256)
@@ -11,7 +21,7 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
                      const float* __restrict__ rgb, const float* __restrict__ m,
                      uint32_t img_w, uint32_t img_h, float* __restrict__ pixxydepth,
                      float* __restrict__ conic, float4* __restrict__ aabb,
-                     float4* __restrict__ splat2, float* __restrict__ zero9) {
+                     float4* __restrict__ splat2, float* __restrict__ zero9, BinOut bo) {
   // the packed records leave through LDS: each wave writes its 64 records
   // (4 KB) with four fully contiguous 1 KB stores instead of four strided
   // quarter-record stores (which leave every line partly written by each
@@ -42,15 +52,22 @@ gauss_project_kernel(uint32_t n, const float* __restrict__ xyz, const float* __r
     ProjState p;
     project_state(m, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q.x, q.y, q.z, q.w, s0, s1,
                   s2, img_w, img_h, p);
-    pixxydepth[3 * i + 0] = p.pix[0];
-    pixxydepth[3 * i + 1] = p.pix[1];
-    pixxydepth[3 * i + 2] = p.ndcz;
+    if (pixxydepth) {
+      pixxydepth[3 * i + 0] = p.pix[0];
+      pixxydepth[3 * i + 1] = p.pix[1];
+      pixxydepth[3 * i + 2] = p.ndcz;
+    }
     if (conic) {  // the packed record holds the conic too
       conic[3 * i + 0] = p.conic[0];
       conic[3 * i + 1] = p.conic[1];
       conic[3 * i + 2] = p.conic[2];
     }
-    aabb[i] = make_float4(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3]);
+    if (aabb) aabb[i] = make_float4(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3]);
+    if (bo.rect) {
+      bo.rect[i] = pack_rect(tile_rect(p.aabb[0], p.aabb[1], p.aabb[2], p.aabb[3], bo.ts, bo.tw,
+                                       bo.th));
+      bo.dkey[i] = depth_key(p.ndcz);
+    }
     if (kPacked) {
       float4* mine = srec + 4 * threadIdx.x;
       mine[0] = make_float4(p.pix[0], p.pix[1], p.conic[0], p.conic[1]);
@@ -115,7 +132,7 @@ int launch_gauss_project(hipStream_t s, uint32_t n, const float* xyz, const floa
                          const float* quat, const float* opacity, const float* rgb,
                          const float* m, uint32_t w, uint32_t h, bool log_scale,
                          float* pixxydepth, float* conic, float* aabb, float* splat2,
-                         float* zero9) {
+                         float* zero9, BinOut bo = BinOut{nullptr, nullptr, 0, 0, 0}) {
   if (n == 0) return DSPLAT_OK;
   const uint32_t nb = div_up(n, 256);
   const float4* q4 = reinterpret_cast<const float4*>(quat);
@@ -123,14 +140,14 @@ int launch_gauss_project(hipStream_t s, uint32_t n, const float* xyz, const floa
   float4* rec = reinterpret_cast<float4*>(splat2);
   if (splat2) {
     if (log_scale)
-      gauss_project_kernel<true, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
+      gauss_project_kernel<true, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9, bo);
     else
-      gauss_project_kernel<false, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
+      gauss_project_kernel<false, true><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9, bo);
   } else {
     if (log_scale)
-      gauss_project_kernel<true, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
+      gauss_project_kernel<true, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9, bo);
     else
-      gauss_project_kernel<false, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9);
+      gauss_project_kernel<false, false><<<nb, 256, 0, s>>>(n, xyz, scale, q4, opacity, rgb, m, w, h, pixxydepth, conic, bb, rec, zero9, bo);
   }
   DSPLAT_HIP(hipGetLastError());
   return DSPLAT_OK;
@@ -174,6 +191,40 @@ extern "C" int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const fl
                               pnt2opacity, pnt2rgb, transform_world2ndc, img_w, img_h,
                               (flags & DSPLAT_SCALE_IS_LOG) != 0, pnt2pixxydepth, pnt2pixconvinv,
                               pnt2pixaabb, pnt2splat2, d_grad9);
+}
+
+extern "C" int dsplat_gauss_project_bin_ws(
+    void* stream, uint32_t num_pnt, const float* pnt2xyz, const float* pnt2scale,
+    const float* pnt2quat, const float* pnt2opacity, const float* pnt2rgb,
+    const float* transform_world2ndc, uint32_t img_w, uint32_t img_h, uint32_t flags,
+    float* pnt2pixxydepth, float* pnt2pixaabb, float* pnt2splat2, float* d_grad9,
+    uint32_t tile_size, uint32_t* tile2idx, uint32_t* idx2pnt, uint32_t capacity,
+    void* workspace, size_t workspace_bytes, uint32_t* d_num_idx) {
+  if (tile_size != 8 && tile_size != 16 && tile_size != 32)
+    return dsplat_set_error(DSPLAT_ERR_ARG, "gauss project_bin: tile_size must be 8, 16 or 32");
+  if ((num_pnt && (!pnt2xyz || !pnt2scale || !pnt2quat || !pnt2opacity || !pnt2rgb ||
+                   !pnt2splat2 || !transform_world2ndc)) ||
+      !tile2idx || !d_num_idx || !workspace || (capacity && !idx2pnt))
+    return dsplat_set_error(DSPLAT_ERR_ARG, "gauss project_bin: null buffer");
+  if (!aligned16(pnt2quat) || !aligned16(pnt2pixaabb) || !aligned16(pnt2splat2))
+    return dsplat_set_error(DSPLAT_ERR_ARG,
+                            "gauss project_bin: quat/aabb/splat2 must be 16-byte aligned");
+  uint32_t tw, th;
+  tile_grid(img_w, img_h, tile_size, &tw, &th);
+  if (tw > 0xFFFFu || th > 0xFFFFu)  // tile rectangles are packed as 16-bit coordinates
+    return dsplat_set_error(DSPLAT_ERR_ARG, "gauss project_bin: more than 65535 tiles along an axis");
+  if (workspace_bytes < bin_workspace_bytes(num_pnt, capacity))
+    return dsplat_set_error(DSPLAT_ERR_CAPACITY, "gauss project_bin: workspace too small");
+  hipStream_t s = as_stream(stream);
+  BinOut bo{nullptr, nullptr, tile_size, tw, th};
+  bin_count_slots(workspace, num_pnt, &bo.rect, &bo.dkey);
+  const int rc = launch_gauss_project(s, num_pnt, pnt2xyz, pnt2scale, pnt2quat, pnt2opacity,
+                                      pnt2rgb, transform_world2ndc, img_w, img_h,
+                                      (flags & DSPLAT_SCALE_IS_LOG) != 0, pnt2pixxydepth,
+                                      nullptr, pnt2pixaabb, pnt2splat2, d_grad9, bo);
+  if (rc) return rc;
+  return bin_after_count(s, num_pnt, tile_size, tw, th, tile2idx, idx2pnt, capacity, workspace,
+                         workspace_bytes, d_num_idx);
 }
 
 extern "C" int dsplat_gauss_project_bwd(void* stream, uint32_t num_pnt, const float* pnt2xyz,

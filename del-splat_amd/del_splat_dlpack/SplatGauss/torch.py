@@ -123,6 +123,40 @@ def project_fwd(xyz, scale, quat, opacity, rgb, m16, img_shape, log_scale=True,
     return pxd, conic, aabb, splat2
 
 
+def project_bin(xyz, scale, quat, opacity, rgb, m16, img_shape, tile_size, capacity,
+                log_scale=True, grad_acc=None, d_num_idx=None, boxes=False, workspace=None):
+    """Projection and binning in one call without a host wait
+    (dsplat_gauss_project_bin_ws): returns (splat2 [N,16], tile2idx [T+1],
+    idx2pnt [capacity], d_num_idx, pixxydepth, aabb). The records, lists and
+    K equal project_fwd followed by bin_gauss_capacity; the projection kernel
+    computes what the binning's count phase would, so pixxydepth and aabb are
+    returned (as (N,3) / (N,4) tensors) only with boxes=True, else None.
+    workspace (uint8, reused across calls) may be passed to avoid an
+    allocation per call."""
+    dev = xyz.device
+    n = xyz.shape[0]
+    w, h = img_shape
+    tw, th = tile_shape(img_shape, tile_size)
+    splat2 = torch.empty((n, 16), device=dev, dtype=torch.float32)
+    pxd = torch.empty((n, 3), device=dev, dtype=torch.float32) if boxes else None
+    aabb = torch.empty((n, 4), device=dev, dtype=torch.float32) if boxes else None
+    if grad_acc is not None:
+        _check_grad_acc(grad_acc, n, dev)
+    ws_bytes = _native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, tw * th, capacity)
+    if workspace is None or workspace.numel() < ws_bytes:
+        workspace = torch.empty(ws_bytes, device=dev, dtype=torch.uint8)
+    tile2idx = torch.empty(tw * th + 1, device=dev, dtype=torch.int32)
+    idx2pnt = torch.empty(max(capacity, 1), device=dev, dtype=torch.int32)
+    if d_num_idx is None:
+        d_num_idx = torch.empty(1, device=dev, dtype=torch.int32)
+    call("dsplat_gauss_project_bin_ws", stream_ptr(dev), n, ptr(xyz), ptr(scale), ptr(quat),
+         ptr(opacity), ptr(rgb), ptr(m16), w, h,
+         _native.DSPLAT_SCALE_IS_LOG if log_scale else 0, ptr(pxd), ptr(aabb), ptr(splat2),
+         ptr(grad_acc), tile_size, ptr(tile2idx), ptr(idx2pnt), capacity, ptr(workspace),
+         workspace.numel(), ptr(d_num_idx))
+    return splat2, tile2idx, idx2pnt[:capacity], d_num_idx, pxd, aabb
+
+
 def _check_grad_acc(grad_acc, n, dev):
     if (grad_acc.dtype != torch.float32 or grad_acc.numel() != 9 * n
             or not grad_acc.is_contiguous() or grad_acc.device != dev):

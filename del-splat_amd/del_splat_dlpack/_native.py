@@ -49,6 +49,7 @@ _SIGS = {
     "dsplat_free": (_int, [_vp, _vp]),
     "dsplat_gauss_rasterize_pnt2splat2": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _vp]),
     "dsplat_gauss_project_fwd": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "dsplat_gauss_project_bin_ws": (_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _sz, _vp]),
     "dsplat_gauss_rasterize_fwd": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_gauss_rasterize_bwd": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "dsplat_gauss_rasterize_fwd_ordered": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

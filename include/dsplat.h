@@ -176,6 +176,27 @@ int dsplat_gauss_project_fwd(void* stream, uint32_t num_pnt, const float* pnt2xy
                              uint32_t flags, float* pnt2pixxydepth, float* pnt2pixconvinv,
                              float* pnt2pixaabb, float* pnt2splat2, float* d_grad9);
 
+/* Projection and binning of the differentiable path in one call, without a
+ * host wait: dsplat_gauss_project_fwd (no conic array) followed by
+ * dsplat_gauss_tile2idx_idx2pnt_ws, except that the projection kernel itself
+ * derives each Gaussian's tile rectangle and depth key, which the binning's
+ * count phase would compute from the boxes and depths. That saves one
+ * kernel, and pnt2pixxydepth and pnt2pixaabb may be null (not written).
+ * Records, lists, ranges and K (*d_num_idx) are identical to the two calls';
+ * the workspace is dsplat_tile2idx_idx2pnt_workspace_size(num_pnt, 0,
+ * capacity) bytes, and K > capacity leaves every list empty as there. No
+ * counterpart in the reference (its binning is splat_gauss.rs:96-195, after
+ * the projection of :6-44). */
+int dsplat_gauss_project_bin_ws(void* stream, uint32_t num_pnt, const float* pnt2xyz,
+                                const float* pnt2scale, const float* pnt2quat,
+                                const float* pnt2opacity, const float* pnt2rgb,
+                                const float* transform_world2ndc, uint32_t img_w,
+                                uint32_t img_h, uint32_t flags, float* pnt2pixxydepth,
+                                float* pnt2pixaabb, float* pnt2splat2, float* d_grad9,
+                                uint32_t tile_size, uint32_t* tile2idx, uint32_t* idx2pnt,
+                                uint32_t capacity, void* workspace, size_t workspace_bytes,
+                                uint32_t* d_num_idx);
+
 /* Tiled raster over packed records, with the state the adjoint needs:
  * pix2rgb [H,W,3], pix2alpha [H,W] (the reference's alpha_sum), pix2nwalk
  * u32 [H,W] (1 + walk position of the last splat that contributed to the

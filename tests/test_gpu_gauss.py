@@ -412,6 +412,49 @@ def test_raster_bwd_fused_l2(ts, threads):
         assert max_rel(a.cpu().numpy(), b.cpu().numpy()) <= 1e-5
 
 
+@pytest.mark.parametrize("n,shape,ts,cap", [(20000, (333, 201), 8, 0), (20000, (333, 201), 16, 0),
+                                             (20000, (640, 480), 32, 0), (1, (64, 48), 16, 0),
+                                             (0, (64, 48), 16, 0), (2000, (128, 128), 16, 10)])
+@pytest.mark.parametrize("boxes", [False, True])
+def test_project_bin_matches_project_then_bin(n, shape, ts, cap, boxes):
+    """dsplat_gauss_project_bin_ws (the projection writes the binning's count
+    outputs itself) against dsplat_gauss_project_fwd followed by
+    dsplat_gauss_tile2idx_idx2pnt_ws on the same inputs: records, K, tile
+    ranges and lists bit-exact, the optional pixel/depth and box outputs too,
+    the cleared accumulators zero; a capacity below K (the last case) leaves
+    every list empty in both."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    s, m = small_scene(max(n, 1), 11, shape)
+    if n == 0:
+        s = {k: v[:0] for k, v in s.items()}
+    xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    m16 = _t(m)
+    pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape, with_conic=False)
+    k_ref = torch.zeros(1, device=DEV, dtype=torch.int32)
+    capacity = cap
+    if capacity == 0:
+        kb = SG.bin_gauss(shape, ts, pxd, aabb)[1].numel()
+        capacity = kb + 17
+    t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, capacity, k_ref)
+    gacc = torch.full((9 * n,), 3.0, device=DEV, dtype=torch.float32)
+    rec, t2i_b, i2p_b, k_b, pxd_b, aabb_b = SG.project_bin(xyz, ls, q, op, rgb, m16, shape, ts,
+                                                           capacity, grad_acc=gacc, boxes=boxes)
+    torch.cuda.synchronize()
+    k = int(k_ref.item())
+    assert int(k_b.item()) == k
+    assert torch.equal(rec, splat2)
+    assert torch.equal(t2i_b, t2i)
+    if k <= capacity:
+        assert torch.equal(i2p_b[:k], i2p[:k])
+    else:
+        assert int(t2i_b.abs().sum().item()) == 0  # overflow: nothing binned
+    if boxes:
+        assert torch.equal(pxd_b, pxd) and torch.equal(aabb_b, aabb)
+    else:
+        assert pxd_b is None and aabb_b is None
+    assert int((gacc != 0).sum().item()) == 0
+
+
 @pytest.mark.parametrize("n", [20000, 1337])
 def test_project_fwd_clears_accumulators_record_only(n):
     """project_fwd without the conic array (with_conic=False) writes the same
