@@ -171,8 +171,8 @@ def cpu_baseline(s, m, shape, ts, reps):
 
 def section_forward(args, torch, np, SG, scene):
     """The plain forward of 100k Gaussians at 512x512 with 16x16 tiles
-    (projection, binning without a host wait, forward raster with RGB and
-    alpha), one frame per step, at the forward workgroup size that is fastest
+    (projection and binning in one call without a host wait, forward raster
+    with RGB and alpha), one frame per step, at the forward workgroup size that is fastest
     on the frame; the raster kernel bracketed by HIP events on the library's
     stream (torch's current stream)."""
     dev = torch.device("cuda:0")
@@ -205,9 +205,14 @@ def section_forward(args, torch, np, SG, scene):
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
 
+    from del_splat_dlpack import _native
+    ws = torch.empty(_native.lib().dsplat_tile2idx_idx2pnt_workspace_size(n, 32 * 32, cap),
+                     device=dev, dtype=torch.uint8)
+
     def step(i, ev=None):
-        pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
-        t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, d_k[i])
+        # projection and binning in one call (no host wait)
+        splat2, t2i, i2p, _, _, _ = SG.project_bin(xyz, ls, q, op, rgb, m16, shape, ts, cap,
+                                                   d_num_idx=d_k[i], workspace=ws)
         if ev:
             ev[0].record()
         out = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, state=False)
