@@ -21,6 +21,7 @@ _u64 = ctypes.c_uint64
 DSPLAT_SCALE_IS_LOG = 1
 DSPLAT_OPACITY_IS_LOGIT = 2
 DSPLAT_GRAD_OVERWRITE = 4
+DSPLAT_ERR_ARG = -1  # status: invalid argument
 DSPLAT_ERR_CAPACITY = -3  # status: workspace, scratch or capacity too small
 
 
