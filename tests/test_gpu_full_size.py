@@ -9,7 +9,10 @@ full-size tests next to their kernels):
   800x800) in one batched forward, per view bit-exact against the
   single-view path (lists and images), and a graph replay equal to it;
 * 3M Gaussians with 8 uneven views: the batched forward + L2 adjoint
-  against the sum of the per-view single-view adjoints.
+  against the sum of the per-view single-view adjoints;
+* the one-call projection and binning the headline frame times (1M
+  Gaussians, 1920x1080, 32x32 tiles) and the 100k-Gaussian forward,
+  bit-exact against the projection followed by the binning.
 
 The single-view path is itself checked against the oracle; at these sizes
 the oracle would take minutes per view, so the batched cases compare
@@ -121,3 +124,23 @@ def test_batched_adjoint_3m_8_views():
     names = dict(xyz="xyz", scale="log_scale", quat="quat", opacity="opacity", rgb="rgb")
     for k, lk in names.items():
         check_grad(g[k].cpu().numpy(), leaf[lk].grad.cpu().numpy(), k, max_rel_tol=1e-3)
+
+
+@pytest.mark.parametrize("n,shape,ts,seed,radius", [(1_000_000, (1920, 1080), 32, 2, 2.0),
+                                                    (100_000, (512, 512), 16, 0, 1.0)])
+def test_project_bin_full_size(n, shape, ts, seed, radius):
+    import del_splat_dlpack.SplatGauss.torch as SG
+    s = scene.gaussians(n, seed, radius=radius)
+    m16 = _t(scene.front_camera(shape))
+    xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
+    pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape, with_conic=False)
+    k = SG.bin_gauss(shape, ts, pxd, aabb)[1].numel()
+    cap = k + k // 8 + 65536
+    k_ref = torch.zeros(1, device=DEV, dtype=torch.int32)
+    t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, k_ref)
+    rec, t2i_b, i2p_b, k_b, _, _ = SG.project_bin(xyz, ls, q, op, rgb, m16, shape, ts, cap)
+    torch.cuda.synchronize()
+    assert int(k_b.item()) == int(k_ref.item()) == k
+    assert torch.equal(rec, splat2)
+    assert torch.equal(t2i_b, t2i)
+    assert torch.equal(i2p_b[:k], i2p[:k])
