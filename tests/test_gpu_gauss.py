@@ -414,7 +414,8 @@ def test_raster_bwd_fused_l2(ts, threads):
 
 @pytest.mark.parametrize("n,shape,ts,cap", [(20000, (333, 201), 8, 0), (20000, (333, 201), 16, 0),
                                              (20000, (640, 480), 32, 0), (1, (64, 48), 16, 0),
-                                             (0, (64, 48), 16, 0), (2000, (128, 128), 16, 10)])
+                                             (0, (64, 48), 16, 0), (2000, (128, 128), 16, 10),
+                                             (500, (1, 1), 8, 0), (3000, (96, 64), 16, -1)])
 @pytest.mark.parametrize("boxes", [False, True])
 def test_project_bin_matches_project_then_bin(n, shape, ts, cap, boxes):
     """dsplat_gauss_project_bin_ws (the projection writes the binning's count
@@ -424,9 +425,14 @@ def test_project_bin_matches_project_then_bin(n, shape, ts, cap, boxes):
     the cleared accumulators zero; a capacity below K (the last case) leaves
     every list empty in both."""
     import del_splat_dlpack.SplatGauss.torch as SG
+    from del_splat_dlpack import camera
     s, m = small_scene(max(n, 1), 11, shape)
     if n == 0:
         s = {k: v[:0] for k, v in s.items()}
+    if cap == -1:
+        # a camera looking away from the scene: every Gaussian culled (w <= 0)
+        m = camera.to_col_major(camera.world2ndc((0.0, 0.0, 3.0), (0.0, 0.0, 6.0), shape))
+        cap = 0
     xyz, ls, q, op, rgb = (_t(s[k]) for k in ("xyz", "log_scale", "quat", "opacity", "rgb"))
     m16 = _t(m)
     pxd, _, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape, with_conic=False)
