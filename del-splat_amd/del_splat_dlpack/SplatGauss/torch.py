@@ -195,10 +195,13 @@ def _check_schedule(tile2idx, *arrays):
             raise ValueError("tile schedule arrays must be int32 with one entry per tile")
 
 
-def tile_order(tile_cost):
+def tile_order(tile_cost, out=None):
     """Tiles by descending cost (int32 [T]): the schedule for rasterize_bwd and,
-    as a hint, the next frame's rasterize_fwd."""
-    order = torch.empty_like(tile_cost)
+    as a hint, the next frame's rasterize_fwd. out (int32 [T]) receives it if
+    given (a graph-captured frame keeps its schedule in one buffer)."""
+    order = torch.empty_like(tile_cost) if out is None else out
+    if order.dtype != torch.int32 or order.numel() != tile_cost.numel():
+        raise ValueError("tile_order: out must be an int32 tensor of the cost's size")
     call("dsplat_tile_order", stream_ptr(tile_cost.device), tile_cost.numel(), ptr(tile_cost),
          ptr(order))
     return order
