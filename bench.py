@@ -252,7 +252,9 @@ def section_sh(args, torch, np, SG, scene):
     rng = np.random.default_rng(6)
     xyz = torch.from_numpy(scene.gaussians(n, 5)["xyz"]).to(dev)
     sh = torch.from_numpy((0.3 * rng.standard_normal((n, 16, 3))).astype(np.float32)).to(dev)
-    cam = (0.0, 0.0, 3.0)
+    # the camera centre lives on the device, as in a training loop (a tuple
+    # would add a pageable host-to-device copy inside each timed kernel)
+    cam = torch.tensor([0.0, 0.0, 3.0], device=dev)
     d_rgb = torch.from_numpy(rng.standard_normal((n, 3)).astype(np.float32)).to(dev)
     d_sh, d_xyz = torch.empty_like(sh), torch.empty_like(xyz)
     steps = max(args.steps, 20)

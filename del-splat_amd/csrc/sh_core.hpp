@@ -89,6 +89,48 @@ __host__ __device__ constexpr int record_stride(int r) {
   return ((r + 3) / 4 * 4) / 4 % 2 == 0 ? (r + 3) / 4 * 4 + 4 : (r + 3) / 4 * 4;
 }
 
+// Number of 16-byte words one lane moves for a full tile of 64 records.
+template <int R>
+constexpr int tile_words() { return (16 * R + 63) / 64; }
+
+// A 16-byte word in registers. A clang vector rather than HIP's float4
+// struct: struct copies become private-memory memcpys that stay in scratch
+// when the words are carried across loop iterations.
+typedef float word4 __attribute__((ext_vector_type(4)));
+
+// Issues the 16-byte loads of a full tile of 64 records (R floats each,
+// contiguous from `src` at record g0) into v, coalesced across the wave; the
+// caller waits for them where it uses v.
+template <int R>
+__device__ inline void tile_load(const float* __restrict__ src, uint32_t g0, int lane,
+                                 word4* v) {
+  const word4* b4 = reinterpret_cast<const word4*>(src + size_t(g0) * R);
+#pragma unroll
+  for (int i = 0; i < tile_words<R>(); ++i) {
+    const int f4 = lane + 64 * i;
+    if (f4 < 16 * R) v[i] = b4[f4];
+  }
+}
+
+// Writes a tile loaded by tile_load into the wave's LDS slice at stride RP.
+template <int R, int RP>
+__device__ inline void tile_to_lds(const word4* v, float* __restrict__ lds, int lane) {
+#pragma unroll
+  for (int i = 0; i < tile_words<R>(); ++i) {
+    const int f4 = lane + 64 * i;
+    if (f4 < 16 * R) {
+      const int f = 4 * f4;
+      if (R % 4 == 0) {
+        *reinterpret_cast<word4*>(lds + (f / R) * RP + f % R) = v[i];
+      } else {
+        const float e[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) lds[((f + j) / R) * RP + (f + j) % R] = e[j];
+      }
+    }
+  }
+}
+
 // Copies the calling wave's records [g0, g0 + cnt) (R floats each,
 // contiguous from `src`) into the wave's LDS slice at stride RP. Full waves
 // move 16-byte words; a tail wave moves single floats so that it never
@@ -96,30 +138,12 @@ __host__ __device__ constexpr int record_stride(int r) {
 template <int R, int RP>
 __device__ inline void stage_in(const float* __restrict__ src, uint32_t g0, uint32_t cnt,
                                 float* __restrict__ lds, int lane) {
-  const float* base = src + size_t(g0) * R;
   if (cnt == 64) {
-    const float4* b4 = reinterpret_cast<const float4*>(base);
-    float4 v[(16 * R + 63) / 64];
-#pragma unroll
-    for (int i = 0; i < (16 * R + 63) / 64; ++i) {
-      const int f4 = lane + 64 * i;
-      if (f4 < 16 * R) v[i] = b4[f4];
-    }
-#pragma unroll
-    for (int i = 0; i < (16 * R + 63) / 64; ++i) {
-      const int f4 = lane + 64 * i;
-      if (f4 < 16 * R) {
-        const int f = 4 * f4;
-        if (R % 4 == 0) {
-          *reinterpret_cast<float4*>(lds + (f / R) * RP + f % R) = v[i];
-        } else {
-          const float e[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
-#pragma unroll
-          for (int j = 0; j < 4; ++j) lds[((f + j) / R) * RP + (f + j) % R] = e[j];
-        }
-      }
-    }
+    word4 v[tile_words<R>()];
+    tile_load<R>(src, g0, lane, v);
+    tile_to_lds<R, RP>(v, lds, lane);
   } else {
+    const float* base = src + size_t(g0) * R;
     const uint32_t total = cnt * R;
     for (uint32_t f = lane; f < total; f += 64) lds[(f / R) * RP + f % R] = base[f];
   }
