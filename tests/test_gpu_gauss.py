@@ -30,6 +30,24 @@ def _project(s, m, shape):
                                        _t(m), shape)
 
 
+def _check_bin(shape, ts, pxd, aabb, o_t2i, o_i2p):
+    """Both binning paths against the oracle's lists: the phased entry points
+    (K read back; depth sort and tile sort) and the one-call workspace entry
+    point sized with the tile count (tile buckets and per-tile sorts), at
+    capacity K and K + 1000."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
+    np.testing.assert_array_equal(u32(t2i), o_t2i)
+    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    k = len(o_i2p)
+    for cap in (k, k + 1000):
+        kbuf = torch.full((1,), -1, device=DEV, dtype=torch.int32)
+        t2i, i2p, _ = SG.bin_gauss_capacity(shape, ts, pxd, aabb, cap, kbuf)
+        assert int(kbuf[0].item()) == k
+        np.testing.assert_array_equal(u32(t2i), o_t2i)
+        np.testing.assert_array_equal(u32(i2p)[:k], o_i2p)
+
+
 @pytest.mark.parametrize("n,shape,seed", [(5000, (256, 192), 0), (20000, (333, 201), 1)])
 def test_projection_parity(oracle, n, shape, seed):
     s, m = small_scene(n, seed, shape)
@@ -53,10 +71,8 @@ def test_binning_bit_exact(oracle, ts, n, shape, seed):
     if n == 0:
         s = {k: v[:0] for k, v in s.items()}
     pxd, conic, aabb = _project(s, m, shape)
-    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
     o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, ts)
-    np.testing.assert_array_equal(u32(t2i), o_t2i)
-    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    _check_bin(shape, ts, pxd, aabb, o_t2i, o_i2p)
 
 
 def test_binning_edge_boxes(oracle):
@@ -76,10 +92,8 @@ def test_binning_edge_boxes(oracle):
     pxd = np.zeros((n, 3), np.float32)
     pxd[:, 2] = np.linspace(-1.5, 1.5, n).astype(np.float32)
     pxd[3, 2] = pxd[4, 2]  # equal depths: ties by index
-    t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, _t(pxd), _t(boxes))
     o_t2i, o_i2p = oracle.tile2pnt(boxes, pxd, shape, ts)
-    np.testing.assert_array_equal(u32(t2i), o_t2i)
-    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    _check_bin(shape, ts, _t(pxd), _t(boxes), o_t2i, o_i2p)
 
 
 def test_binning_allocating_variant(oracle):
@@ -129,23 +143,26 @@ def test_binning_one_call_and_count_only(oracle):
     assert i2p.numel() == 0 and int(t2i.abs().sum().item()) == 0
 
 
-@pytest.mark.parametrize("n,shape", [(5000, (64, 64)), (20000, (64, 64)), (20000, (40, 30))])
-def test_binning_dense_tiles_and_equal_depths(oracle, n, shape):
+@pytest.mark.parametrize("n,shape,ties", [(5000, (64, 64), True), (20000, (64, 64), True),
+                                          (20000, (40, 30), True), (12000, (64, 64), False)])
+def test_binning_dense_tiles_and_equal_depths(oracle, n, shape, ties):
     """Every box covers the whole image, so each tile's list holds (nearly)
-    all n Gaussians; a third of the depths repeat three values, so long runs
-    of equal depth keys must keep ascending index order."""
-    import del_splat_dlpack.SplatGauss.torch as SG
+    all n Gaussians (beyond the 8192 pairs a tile sort keeps in LDS from n =
+    12000 on); with ties, a third of the depths repeat three values, so long
+    runs of equal depth keys must keep ascending index order."""
     rng = np.random.default_rng(n + shape[0])
     boxes = np.tile(np.array([[-1.0, -1.0, shape[0] + 1.0, shape[1] + 1.0]], np.float32), (n, 1))
     boxes[::7, 0] = 10.0  # some boxes start one tile in
     pxd = np.zeros((n, 3), np.float32)
-    pxd[:, 2] = rng.uniform(-0.9, 0.9, n).astype(np.float32)
-    pxd[::3, 2] = np.float32([0.1, 0.2, 0.3])[rng.integers(0, 3, len(pxd[::3]))]
+    if ties:
+        pxd[:, 2] = rng.uniform(-0.9, 0.9, n).astype(np.float32)
+        pxd[::3, 2] = np.float32([0.1, 0.2, 0.3])[rng.integers(0, 3, len(pxd[::3]))]
+    else:
+        pxd[:, 2] = rng.permutation(np.linspace(-0.9, 0.9, n)).astype(np.float32)
+        assert len(np.unique(pxd[:, 2])) == n
     for ts in (16, 32):
-        t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, _t(pxd), _t(boxes))
         o_t2i, o_i2p = oracle.tile2pnt(boxes, pxd, shape, ts)
-        np.testing.assert_array_equal(u32(t2i), o_t2i)
-        np.testing.assert_array_equal(u32(i2p), o_i2p)
+        _check_bin(shape, ts, _t(pxd), _t(boxes), o_t2i, o_i2p)
 
 
 def test_binning_large_tile_grid(oracle):
@@ -155,10 +172,8 @@ def test_binning_large_tile_grid(oracle):
     shape = (1024, 1024)
     s, m = small_scene(20000, 8, shape)
     pxd, conic, aabb = _project(s, m, shape)
-    t2i, i2p = SG.tile2idx_idx2pnt(shape, 8, pxd, aabb)
     o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, 8)
-    np.testing.assert_array_equal(u32(t2i), o_t2i)
-    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    _check_bin(shape, 8, pxd, aabb, o_t2i, o_i2p)
 
 
 def test_binning_many_gaussians(oracle):
@@ -168,11 +183,9 @@ def test_binning_many_gaussians(oracle):
     shape = (320, 240)
     s, m = small_scene(4_500_000, 9, shape)
     pxd, conic, aabb = _project(s, m, shape)
-    t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, pxd, aabb)
     o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, 16)
     print(f"pairs {len(o_i2p)}", flush=True)
-    np.testing.assert_array_equal(u32(t2i), o_t2i)
-    np.testing.assert_array_equal(u32(i2p), o_i2p)
+    _check_bin(shape, 16, pxd, aabb, o_t2i, o_i2p)
 
 
 def _copy_device(dst, src, nbytes):
