@@ -81,7 +81,16 @@ __device__ __forceinline__ ConicQ conic_q(float a, float b, float c) {
 }
 
 // exp(-q/2) where in, else 0 (2^-inf = 0: a select on the argument keeps
-// the exp unpredicated).
+// the exp unpredicated). cyy = (c dy) dy, the last term of q, passed in by
+// callers that share it between pixels of one row (the same roundings).
+__device__ __forceinline__ float splat_weight_cyy(bool in, const ConicQ& k, float dx, float dy,
+                                                  float cyy) {
+#pragma clang fp contract(off)
+  constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
+  const float q = ((k.a * dx) * dx + k.b2 * (dx * dy)) + cyy;
+  return __builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff());
+}
+
 __device__ __forceinline__ float splat_weight(bool in, const ConicQ& k, float dx, float dy) {
 #pragma clang fp contract(off)
   constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
@@ -108,6 +117,10 @@ struct GroupMap {
   // 8 x 8 regions, one pixel per lane: a list entry's box test for a region
   // is a 64-bit lane mask (region_lane_mask) formed in scalar registers
   static constexpr bool kLaneMask = RS == 8;
+  // One pixel per lane and a wave's G regions side by side in one row of
+  // regions (G divides the row): all of a lane's pixels share their pixel
+  // row, so an entry's dy is the same for every group.
+  static constexpr bool kRowShared = Q == 1 && G > 1 && RXN % G == 0;
   static_assert(TS % RS == 0 && G >= 1 && G * W == RXN * RXN, "unsupported tile / workgroup shape");
   __device__ static uint32_t wave() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
   __device__ static uint32_t rx0(int g) { return ((wave() * G + g) % RXN) * RS; }
@@ -727,10 +740,15 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       const float4 q2 = s2[j], q3 = s3[j];
       const float4 q0 = s0[j], q1 = s1[j];
       const ConicQ ke{q0.z, q0.w, q1.x};  // b doubled at staging
+      // one row for all of the lane's pixels: dy and (c dy) dy once per
+      // entry (in the forward, hoisting them measured slower)
+      const float dyr = py[0] - q0.y;
+      const float cyy = (ke.c * dyr) * dyr;
       // -0 so that the first group's terms enter without an add (x + -0 = x)
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = -0.f;
+      float wsum = -0.f;  // sum of -2 dL/dq over the lane's groups (kRowShared)
       bool touched = false;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -750,8 +768,10 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
         for (int qq = 0; qq < Q; ++qq) {
           const int k = g * Q + qq;
           const bool in = inm[qq];
-          const float dx = px[k] - q0.x, dy = py[k] - q0.y;
-          const float gk = splat_weight(in, ke, dx, dy);
+          const float dx = px[k] - q0.x;
+          const float dy = GM::kRowShared ? dyr : py[k] - q0.y;
+          const float gk = GM::kRowShared ? splat_weight_cyy(in, ke, dx, dy, cyy)
+                                          : splat_weight(in, ke, dx, dy);
           const float e = gk * q1.y;
           const float ome = 1.f - e;
           const float inv = __builtin_amdgcn_rcpf(ome);
@@ -771,15 +791,27 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           // the entry are linear in (S_x, S_y, S_xx, S_xy, S_yy), formed once
           // at the flush (where the factor -1/2 is applied, exactly)
           const float dl_dq2 = dl_de * e;
-          const float a = dl_dq2 * dx, bq = dl_dq2 * dy;
+          const float a = dl_dq2 * dx;
           v[0] = v[0] + a;
-          v[1] = v[1] + bq;
           v[2] = fmaf(a, dx, v[2]);
-          v[3] = fmaf(a, dy, v[3]);
-          v[4] = fmaf(bq, dy, v[4]);
+          if constexpr (GM::kRowShared) {
+            // dy is the lane's for every group: the y moments follow from
+            // sum w and sum w dx after the groups
+            wsum = wsum + dl_dq2;
+          } else {
+            const float bq = dl_dq2 * dy;
+            v[1] = v[1] + bq;
+            v[3] = fmaf(a, dy, v[3]);
+            v[4] = fmaf(bq, dy, v[4]);
+          }
         }
       }
       if (!touched) continue;
+      if constexpr (GM::kRowShared) {
+        v[1] = wsum * dyr;
+        v[3] = v[0] * dyr;
+        v[4] = v[1] * dyr;
+      }
       float tot, tot9;
       wave_sum9_scatter(v, lane, tot, tot9);
       // lanes 0, 8, ..., 56 hold the eight paired totals, lane 1 writes the
