@@ -8,8 +8,14 @@ namespace dsplat {
 // tile_order (may be null: raster order) maps workgroups to tiles, so the
 // longest walks start first; tile_cost (may be null) receives each tile's
 // longest walk.
+// Minimum waves per SIMD the compiler must keep for a forward shape: the
+// headline's 32x32 tile at 512 threads runs four workgroups per CU only
+// within 64 VGPRs (8 waves per SIMD); other shapes are left to the compiler.
+template <int TS, int NT>
+constexpr int fwd_min_waves() { return TS == 32 && NT == 512 ? 8 : 1; }
+
 template <int TS, int NT, typename Loader, bool kState>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(fwd_min_waves<TS, NT>())))
 gauss_raster_fwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw, Loader ld,
                         const uint32_t* __restrict__ tile2idx,
                         const uint32_t* __restrict__ idx2pnt, float* __restrict__ pix2rgb,

@@ -374,7 +374,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
       s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
       s2[threadIdx.x] = q2;
-      s3[threadIdx.x] = q3;
+      // the record's free second slot carries the entry's walk position + 1
+      // (what a pixel it reaches records as nwalk): read from LDS it is a
+      // vector operand, so the per-pixel select needs no scalar-to-vector move
+      s3[threadIdx.x] = make_float4(q3.x, __uint_as_float(it + 1u), q3.z, q3.w);
       smask.stage(threadIdx.x, tx, ty, q2.y, q2.z, q2.w, q3.x);
     }
     __syncthreads();
@@ -422,6 +425,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
           const float4 q0 = s0[j];
           const float4 q1 = s1[j];
           const ConicQ ke{q0.z, q0.w, q1.x};  // b doubled at staging
+          const uint32_t wpos = __float_as_uint(q3.y);  // b + j + 1
 #pragma unroll
           for (int g = 0; g < G; ++g) {
             bool inm[Q];
@@ -442,7 +446,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
                 if constexpr (kState) {
                   tl[k] = in ? tr[k] : tl[k];
-                  walked[k] = in ? b + j + 1 : walked[k];
+                  walked[k] = in ? wpos : walked[k];
                 }
                 tr[k] = tr[k] * (1.f - e);
                 asum[k] = asum[k] + eo;
