@@ -752,7 +752,6 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       float v[kGrad];
 #pragma unroll
       for (int c = 0; c < kGrad; ++c) v[c] = -0.f;
-      float wsum = -0.f;  // sum of -2 dL/dq over the lane's groups (kRowShared)
       bool touched = false;
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -800,8 +799,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           v[2] = fmaf(a, dx, v[2]);
           if constexpr (GM::kRowShared) {
             // dy is the lane's for every group: the y moments follow from
-            // sum w and sum w dx after the groups
-            wsum = wsum + dl_dq2;
+            // sum w (= opacity * v[5]) and sum w dx after the groups
           } else {
             const float bq = dl_dq2 * dy;
             v[1] = v[1] + bq;
@@ -812,7 +810,8 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       }
       if (!touched) continue;
       if constexpr (GM::kRowShared) {
-        v[1] = wsum * dyr;
+        // sum of dL/de e = opacity * sum of dL/de g (e = g * opacity)
+        v[1] = (v[5] * q1.y) * dyr;
         v[3] = v[0] * dyr;
         v[4] = v[1] * dyr;
       }
