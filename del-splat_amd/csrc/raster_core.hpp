@@ -605,7 +605,7 @@ struct GradSink {
 // take so much LDS that fewer workgroups fit on a CU).
 template <int TS, int NT>
 struct BwdLds {
-  static constexpr int B = NT < 128 ? NT : 128;
+  static constexpr int B = 64;
   static constexpr int S = NT / 64 <= 2 ? NT / 64 : 1;
   float4 s0[B], s1[B], s2[B], s3[B];
   uint32_t spid[B];
