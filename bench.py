@@ -126,14 +126,14 @@ def cpu_baseline(s, m, shape, ts, reps):
         t1 = time.perf_counter()
         t2i, i2p = O.tile2pnt_gauss(pxd, aabb, shp, t)
         t2 = time.perf_counter()
-        img, alpha, nwalk, tlast = O.gauss_rasterize(shp, pxd, conic, aabb, S["opacity"],
+        img, alpha, nwalk, tfinal = O.gauss_rasterize(shp, pxd, conic, aabb, S["opacity"],
                                                      S["rgb"], t, t2i, i2p)
         t3 = time.perf_counter()
         st.update(project=t1 - t0, bin=t2 - t1, raster=t3 - t2)
         if adjoint:
             d_img = (2.0 * (img - tgt)).astype(np.float32)
             f = O.gauss_rasterize_bwd if threads == 1 else O.gauss_rasterize_bwd_mt
-            d = f(shp, pxd, conic, aabb, S["opacity"], S["rgb"], t, t2i, i2p, nwalk, tlast,
+            d = f(shp, pxd, conic, aabb, S["opacity"], S["rgb"], t, t2i, i2p, nwalk, tfinal,
                   d_img)
             O.gauss_project_bwd(S["xyz"], S["quat"], S["scale"], M, shp, d[0], d[1],
                                 log_scale=True)
@@ -446,14 +446,14 @@ def headline(args, torch, np, SG, scene):
             mark("project_bin", False)
         mark("raster_fwd", True)
         cost = torch.empty(tw * th, device=dev, dtype=torch.int32)
-        img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads,
+        img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, args.threads,
                                                     tile_order=hint[0], tile_cost=cost)
         order = SG.tile_order(cost)
         hint[0] = order
         mark("raster_fwd", False)
         # the L2 loss gradient 2 (img - target) is formed inside the adjoint
         mark("raster_bwd", True)
-        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
+        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tfinal,
                                                     img, target, args.threads_bwd,
                                                     tile_order=order, grad_acc=gacc)
         mark("raster_bwd", False)
@@ -524,11 +524,11 @@ def batched(args, torch, np, SG, scene):
             for v, (_, shape) in enumerate(views):
                 pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, mats[v], shape)
                 t2i, i2p = SG.bin_gauss(shape, pts, pxd, aabb)
-                img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, pts, splat2, t2i, i2p, pth)
+                img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, pts, splat2, t2i, i2p, pth)
                 if mode == "fwd_bwd":
                     d_img = 2.0 * (img - targets[v])
                     dpix, dcon, dop, drgb = SG.rasterize_bwd(shape, pts, splat2, t2i, i2p, nwalk,
-                                                             tlast, d_img, None, pthb)
+                                                             tfinal, d_img, None, pthb)
                     SG.project_bwd(xyz, ls, q, mats[v], shape, dpix, dcon)
 
         def per_view_rate(cfg, nrep):

@@ -294,7 +294,7 @@ mv_raster_fwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
                      const float4* __restrict__ rec, const uint32_t* __restrict__ tile2idx,
                      const uint32_t* __restrict__ idx2pnt, float* __restrict__ rgb,
                      float* __restrict__ alpha, uint32_t* __restrict__ nwalk,
-                     float* __restrict__ tlast, uint32_t* __restrict__ counter,
+                     float* __restrict__ tfinal, uint32_t* __restrict__ counter,
                      const uint32_t* __restrict__ tile_order, uint32_t* __restrict__ tile_cost) {
   const PackedLoader ld{rec};
   for (;;) {
@@ -308,7 +308,7 @@ mv_raster_fwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
     const size_t po = vd.pix_off;
     raster_tile_fwd<TS, NT, PackedLoader>(local % vd.tw, local / vd.tw, vd.w, vd.h, tile2idx[g],
                                           tile2idx[g + 1], ld, idx2pnt, rgb + 3 * po, alpha + po,
-                                          nwalk + po, tlast + po, tile_cost, g);
+                                          nwalk + po, tfinal + po, tile_cost, g);
   }
 }
 
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(NT)
 mv_raster_bwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t n,
                      uint32_t total_tiles, const float4* __restrict__ rec,
                      const uint32_t* __restrict__ tile2idx, const uint32_t* __restrict__ idx2pnt,
-                     const uint32_t* __restrict__ nwalk, const float* __restrict__ tlast,
+                     const uint32_t* __restrict__ nwalk, const float* __restrict__ tfinal,
                      const float* __restrict__ d_img, const float* __restrict__ rgb,
                      const float* __restrict__ target, const float* __restrict__ d_alpha,
                      GradSink out, uint32_t* __restrict__ counter,
@@ -332,7 +332,7 @@ mv_raster_bwd_kernel(const ViewDev* __restrict__ views, uint32_t nview, uint32_t
     const uint32_t local = g - vd.tile_off;
     const size_t po = vd.pix_off;
     raster_tile_bwd<TS, NT>(local % vd.tw, local / vd.tw, vd.w, vd.h, tile2idx[g + 1], rec,
-                            idx2pnt, nwalk + po, tlast + po, d_img ? d_img + 3 * po : nullptr,
+                            idx2pnt, nwalk + po, tfinal + po, d_img ? d_img + 3 * po : nullptr,
                             rgb + 3 * po, target ? target + 3 * po : nullptr,
                             d_alpha ? d_alpha + po : nullptr, v * n, out);
   }
@@ -453,7 +453,7 @@ struct dsplat_mv_plan {
   void* tsort_ws = nullptr;
   size_t tsort_bytes = 0;
   uint32_t *tile2idx = nullptr, *misc = nullptr;
-  float *rgb = nullptr, *alpha = nullptr, *tlast = nullptr;
+  float *rgb = nullptr, *alpha = nullptr, *tfinal = nullptr;
   uint32_t* nwalk = nullptr;
   float *d_pixxy = nullptr, *d_conic = nullptr;
   // per global tile: the last forward's longest walk, and the tiles ordered by
@@ -598,7 +598,7 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   const size_t o_t2i = take((size_t(P->max_tiles) + 1) * 4);
   const size_t o_misc = take(64);
   const size_t o_rgb = take(max_pixels * 12), o_alpha = take(max_pixels * 4);
-  const size_t o_nwalk = take(max_pixels * 4), o_tlast = take(max_pixels * 4);
+  const size_t o_nwalk = take(max_pixels * 4), o_tfinal = take(max_pixels * 4);
   const size_t o_dpix = take(items * 8), o_dcon = take(items * 12);
   const size_t o_tcost = take(size_t(P->max_tiles) * 4), o_tord = take(size_t(P->max_tiles) * 4);
   P->mem_bytes = o;
@@ -625,7 +625,7 @@ extern "C" int dsplat_mv_plan_create(uint32_t num_pnt, uint32_t max_view, uint64
   P->rgb = reinterpret_cast<float*>(b + o_rgb);
   P->alpha = reinterpret_cast<float*>(b + o_alpha);
   P->nwalk = reinterpret_cast<uint32_t*>(b + o_nwalk);
-  P->tlast = reinterpret_cast<float*>(b + o_tlast);
+  P->tfinal = reinterpret_cast<float*>(b + o_tfinal);
   P->d_pixxy = reinterpret_cast<float*>(b + o_dpix);
   P->d_conic = reinterpret_cast<float*>(b + o_dcon);
   P->tile_cost = reinterpret_cast<uint32_t*>(b + o_tcost);
@@ -737,13 +737,13 @@ extern "C" int dsplat_mv_plan_info(const dsplat_mv_plan* P, uint32_t* num_view,
 }
 
 extern "C" int dsplat_mv_outputs(const dsplat_mv_plan* P, float** rgb, float** alpha,
-                                 uint32_t** nwalk, float** tlast, uint32_t** tile2idx,
+                                 uint32_t** nwalk, float** tfinal, uint32_t** tile2idx,
                                  uint32_t** idx2pnt, uint32_t** counters) {
   if (!P) return dsplat_set_error(DSPLAT_ERR_ARG, "multiview: null plan");
   if (rgb) *rgb = P->rgb;
   if (alpha) *alpha = P->alpha;
   if (nwalk) *nwalk = P->nwalk;
-  if (tlast) *tlast = P->tlast;
+  if (tfinal) *tfinal = P->tfinal;
   if (tile2idx) *tile2idx = P->tile2idx;
   if (idx2pnt) *idx2pnt = P->idx2pnt;
   if (counters) *counters = P->misc;
@@ -760,7 +760,7 @@ template <int TS, int NT>
 static void launch_mv_fwd(dsplat_mv_plan* P, hipStream_t s, float* out_rgb, float* out_alpha) {
   mv_raster_fwd_kernel<TS, NT><<<P->persist_fwd, NT, 0, s>>>(
       P->d_views, P->nview, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, out_rgb, out_alpha,
-      P->nwalk, P->tlast, P->misc + 2, P->tile_order, P->tile_cost);
+      P->nwalk, P->tfinal, P->misc + 2, P->tile_order, P->tile_cost);
 }
 
 template <int TS, int NT>
@@ -769,7 +769,7 @@ static void launch_mv_bwd(dsplat_mv_plan* P, hipStream_t s, const float* out_rgb
                           GradSink out) {
   mv_raster_bwd_kernel<TS, NT><<<P->persist_bwd, NT, 0, s>>>(
       P->d_views, P->nview, P->n, P->total_tiles, P->rec, P->tile2idx, P->idx2pnt, P->nwalk,
-      P->tlast, d_img, out_rgb, target, d_alpha, out, P->misc + 3, P->tile_order);
+      P->tfinal, d_img, out_rgb, target, d_alpha, out, P->misc + 3, P->tile_order);
 }
 
 template <int K, int DEG>

@@ -20,13 +20,13 @@ gauss_raster_fwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw, Loader ld,
                         const uint32_t* __restrict__ tile2idx,
                         const uint32_t* __restrict__ idx2pnt, float* __restrict__ pix2rgb,
                         float* __restrict__ pix2alpha, uint32_t* __restrict__ pix2nwalk,
-                        float* __restrict__ pix2tlast, const uint32_t* __restrict__ tile_order,
+                        float* __restrict__ pix2tfinal, const uint32_t* __restrict__ tile_order,
                         uint32_t* __restrict__ tile_cost) {
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   if (tile >= gridDim.x) return;  // uniform: an order entry outside the grid is skipped
   raster_tile_fwd<TS, NT, Loader, kState>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile],
                                   tile2idx[tile + 1], ld, idx2pnt, pix2rgb, pix2alpha, pix2nwalk,
-                                  pix2tlast, tile_cost, tile);
+                                  pix2tfinal, tile_cost, tile);
 }
 
 template <int TS, int NT>
@@ -35,7 +35,7 @@ gauss_raster_bwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw,
                         const float4* __restrict__ splat2, const uint32_t* __restrict__ tile2idx,
                         const uint32_t* __restrict__ idx2pnt,
                         const uint32_t* __restrict__ pix2nwalk,
-                        const float* __restrict__ pix2tlast, const float* __restrict__ d_img,
+                        const float* __restrict__ pix2tfinal, const float* __restrict__ d_img,
                         const float* __restrict__ pix2rgb, const float* __restrict__ target,
                         const float* __restrict__ d_alpha, float* __restrict__ d_pixxy,
                         float* __restrict__ d_conic, float* __restrict__ d_op,
@@ -44,7 +44,7 @@ gauss_raster_bwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw,
   if (tile >= gridDim.x) return;  // uniform: an order entry outside the grid is skipped
   const GradSink out{d_pixxy, d_conic, d_op, d_rgb};
   raster_tile_bwd<TS, NT>(tile % tw, tile / tw, img_w, img_h, tile2idx[tile + 1], splat2,
-                          idx2pnt, pix2nwalk, pix2tlast, d_img, pix2rgb, target, d_alpha, 0u,
+                          idx2pnt, pix2nwalk, pix2tfinal, d_img, pix2rgb, target, d_alpha, 0u,
                           out);
 }
 
@@ -52,22 +52,22 @@ gauss_raster_bwd_kernel(uint32_t img_w, uint32_t img_h, uint32_t tw,
 template <typename Loader>
 int launch_fwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t threads,
                const Loader& ld, const uint32_t* tile2idx, const uint32_t* idx2pnt, float* rgb,
-               float* alpha, uint32_t* nwalk, float* tlast, const uint32_t* order = nullptr,
+               float* alpha, uint32_t* nwalk, float* tfinal, const uint32_t* order = nullptr,
                uint32_t* cost = nullptr) {
   uint32_t tw, th;
   tile_grid(w, h, ts, &tw, &th);
   const uint32_t nt = tw * th;
   if (nt == 0) return DSPLAT_OK;
   // the adjoint's state and the tile costs only when asked for
-  const bool state = nwalk || tlast || cost;
+  const bool state = nwalk || tfinal || cost;
 #define DSPLAT_FWD(TS_, NT_)                                                                     \
   do {                                                                                           \
     if (state)                                                                                   \
       gauss_raster_fwd_kernel<TS_, NT_, Loader, true><<<nt, NT_, 0, s>>>(                        \
-          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tlast, order, cost);                \
+          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tfinal, order, cost);                \
     else                                                                                         \
       gauss_raster_fwd_kernel<TS_, NT_, Loader, false><<<nt, NT_, 0, s>>>(                       \
-          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tlast, order, cost);                \
+          w, h, tw, ld, tile2idx, idx2pnt, rgb, alpha, nwalk, tfinal, order, cost);                \
   } while (0)
   if (ts == 8) {
     DSPLAT_FWD(8, 64);
@@ -90,7 +90,7 @@ int launch_fwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
 
 int launch_bwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t threads,
                const float4* splat2, const uint32_t* tile2idx, const uint32_t* idx2pnt,
-               const uint32_t* nwalk, const float* tlast, const float* d_img,
+               const uint32_t* nwalk, const float* tfinal, const float* d_img,
                const float* d_alpha, float* d_pixxy, float* d_conic, float* d_op, float* d_rgb,
                const uint32_t* order = nullptr, const float* rgb = nullptr,
                const float* target = nullptr) {
@@ -100,7 +100,7 @@ int launch_bwd(hipStream_t s, uint32_t w, uint32_t h, uint32_t ts, uint32_t thre
   if (nt == 0) return DSPLAT_OK;
 #define DSPLAT_BWD(TS_, NT_)                                                            \
   gauss_raster_bwd_kernel<TS_, NT_><<<nt, NT_, 0, s>>>(w, h, tw, splat2, tile2idx, idx2pnt, \
-                                                       nwalk, tlast, d_img, rgb, target,      \
+                                                       nwalk, tfinal, d_img, rgb, target,      \
                                                        d_alpha, d_pixxy, d_conic, d_op, d_rgb, \
                                                        order)
   if (ts == 8) {
@@ -150,32 +150,32 @@ extern "C" int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t
                                           const float* pnt2splat2, const uint32_t* tile2idx,
                                           const uint32_t* idx2pnt, float* pix2rgb,
                                           float* pix2alpha, uint32_t* pix2nwalk,
-                                          float* pix2tlast) {
+                                          float* pix2tfinal) {
   if (!pix2rgb || !tile2idx || !pnt2splat2)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_fwd: null buffer");
   if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_fwd: splat2 must be 16-byte aligned");
   PackedLoader ld{reinterpret_cast<const float4*>(pnt2splat2)};
   return launch_fwd(as_stream(stream), img_w, img_h, tile_size, threads, ld, tile2idx, idx2pnt,
-                    pix2rgb, pix2alpha, pix2nwalk, pix2tlast);
+                    pix2rgb, pix2alpha, pix2nwalk, pix2tfinal);
 }
 
 extern "C" int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                           uint32_t tile_size, uint32_t threads,
                                           const float* pnt2splat2, const uint32_t* tile2idx,
                                           const uint32_t* idx2pnt, const uint32_t* pix2nwalk,
-                                          const float* pix2tlast, const float* d_pix2rgb,
+                                          const float* pix2tfinal, const float* d_pix2rgb,
                                           const float* d_pix2alpha, float* d_pnt2pixxy,
                                           float* d_pnt2conic, float* d_pnt2opacity,
                                           float* d_pnt2rgb) {
-  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tlast || !d_pix2rgb || !d_pnt2pixxy ||
+  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tfinal || !d_pix2rgb || !d_pnt2pixxy ||
       !d_pnt2conic || !d_pnt2opacity || !d_pnt2rgb)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd: null buffer");
   if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd: splat2 must be 16-byte aligned");
   return launch_bwd(as_stream(stream), img_w, img_h, tile_size, threads,
                     reinterpret_cast<const float4*>(pnt2splat2), tile2idx, idx2pnt, pix2nwalk,
-                    pix2tlast, d_pix2rgb, d_pix2alpha, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
+                    pix2tfinal, d_pix2rgb, d_pix2alpha, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
                     d_pnt2rgb);
 }
 
@@ -186,46 +186,46 @@ extern "C" int dsplat_gauss_rasterize_fwd_ordered(void* stream, uint32_t img_w, 
                                                   const uint32_t* idx2pnt,
                                                   const uint32_t* tile_order, float* pix2rgb,
                                                   float* pix2alpha, uint32_t* pix2nwalk,
-                                                  float* pix2tlast, uint32_t* tile_cost) {
+                                                  float* pix2tfinal, uint32_t* tile_cost) {
   if (!pix2rgb || !tile2idx || !pnt2splat2)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_fwd: null buffer");
   if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_fwd: splat2 must be 16-byte aligned");
   PackedLoader ld{reinterpret_cast<const float4*>(pnt2splat2)};
   return launch_fwd(as_stream(stream), img_w, img_h, tile_size, threads, ld, tile2idx, idx2pnt,
-                    pix2rgb, pix2alpha, pix2nwalk, pix2tlast, tile_order, tile_cost);
+                    pix2rgb, pix2alpha, pix2nwalk, pix2tfinal, tile_order, tile_cost);
 }
 
 extern "C" int dsplat_gauss_rasterize_bwd_ordered(
     void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size, uint32_t threads,
     const float* pnt2splat2, const uint32_t* tile2idx, const uint32_t* idx2pnt,
-    const uint32_t* tile_order, const uint32_t* pix2nwalk, const float* pix2tlast,
+    const uint32_t* tile_order, const uint32_t* pix2nwalk, const float* pix2tfinal,
     const float* d_pix2rgb, const float* d_pix2alpha, float* d_pnt2pixxy, float* d_pnt2conic,
     float* d_pnt2opacity, float* d_pnt2rgb) {
-  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tlast || !d_pix2rgb || !d_pnt2pixxy ||
+  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tfinal || !d_pix2rgb || !d_pnt2pixxy ||
       !d_pnt2conic || !d_pnt2opacity || !d_pnt2rgb)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd: null buffer");
   if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd: splat2 must be 16-byte aligned");
   return launch_bwd(as_stream(stream), img_w, img_h, tile_size, threads,
                     reinterpret_cast<const float4*>(pnt2splat2), tile2idx, idx2pnt, pix2nwalk,
-                    pix2tlast, d_pix2rgb, d_pix2alpha, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
+                    pix2tfinal, d_pix2rgb, d_pix2alpha, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
                     d_pnt2rgb, tile_order);
 }
 
 extern "C" int dsplat_gauss_rasterize_bwd_l2(
     void* stream, uint32_t img_w, uint32_t img_h, uint32_t tile_size, uint32_t threads,
     const float* pnt2splat2, const uint32_t* tile2idx, const uint32_t* idx2pnt,
-    const uint32_t* tile_order, const uint32_t* pix2nwalk, const float* pix2tlast,
+    const uint32_t* tile_order, const uint32_t* pix2nwalk, const float* pix2tfinal,
     const float* pix2rgb, const float* target, float* d_pnt2pixxy, float* d_pnt2conic,
     float* d_pnt2opacity, float* d_pnt2rgb) {
-  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tlast || !pix2rgb || !target ||
+  if (!pnt2splat2 || !tile2idx || !pix2nwalk || !pix2tfinal || !pix2rgb || !target ||
       !d_pnt2pixxy || !d_pnt2conic || !d_pnt2opacity || !d_pnt2rgb)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd_l2: null buffer");
   if (reinterpret_cast<uintptr_t>(pnt2splat2) & 15u)
     return dsplat_set_error(DSPLAT_ERR_ARG, "rasterize_bwd_l2: splat2 must be 16-byte aligned");
   return launch_bwd(as_stream(stream), img_w, img_h, tile_size, threads,
                     reinterpret_cast<const float4*>(pnt2splat2), tile2idx, idx2pnt, pix2nwalk,
-                    pix2tlast, nullptr, nullptr, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
+                    pix2tfinal, nullptr, nullptr, d_pnt2pixxy, d_pnt2conic, d_pnt2opacity,
                     d_pnt2rgb, tile_order, pix2rgb, target);
 }

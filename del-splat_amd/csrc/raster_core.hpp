@@ -18,10 +18,11 @@
 //
 // State for the adjoint: nwalk = 1 + list position of the last splat that
 // contributed to the pixel (the terminating one if the walk terminated, 0 if
-// none) and tlast = the transmittance in front of that splat.
+// none) and tfinal = the pixel's transmittance after its walk (behind that
+// splat; 1 if none).
 //
 // Adjoint: every pixel walks its list back-to-front from position nwalk - 1
-// with the transmittance recovered by division from tlast (dividing by 1 - e
+// with the transmittance recovered by division from tfinal (dividing by 1 - e
 // of earlier splats is safe because the walk did not stop there, so T stayed
 // above ~1e-3). Per list entry the nine gradient terms of a wave's pixels
 // are summed across the wave with DPP, across waves through per-wave LDS
@@ -318,9 +319,9 @@ __device__ __forceinline__ bool box_take(int g, const bool (&walking)[GroupMap<T
 }
 
 // Forward of one tile. Image arrays are the view's base pointers (pixel
-// index iy * img_w + ix); alpha / nwalk / tlast may be null. The entry loop is
+// index iy * img_w + ix); alpha / nwalk / tfinal may be null. The entry loop is
 // outermost so one LDS read of a record serves all PPT pixels of a thread.
-// kState: track the adjoint's state (nwalk, tlast) and the tile cost; a
+// kState: track the adjoint's state (nwalk, tfinal) and the tile cost; a
 // forward asked for neither leaves their two selects per pixel and entry out.
 template <int TS, int NT, typename Loader, bool kState = true>
 __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32_t img_w,
@@ -330,7 +331,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                                                 float* __restrict__ pix2rgb,
                                                 float* __restrict__ pix2alpha,
                                                 uint32_t* __restrict__ pix2nwalk,
-                                                float* __restrict__ pix2tlast,
+                                                float* __restrict__ pix2tfinal,
                                                 uint32_t* __restrict__ tile_cost = nullptr,
                                                 uint32_t cost_slot = 0) {
 #pragma clang fp contract(off)
@@ -345,7 +346,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
   const uint32_t gfirst = GM::wave() * uint32_t(G);
   // A pixel is live while !(alpha_sum >= 0.999f) (NaN stays live, as in the
   // reference's test); pixels outside the image start at alpha_sum = 1.
-  float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], tl[PPT], px[PPT], py[PPT];
+  float cr[PPT], cg[PPT], cbl[PPT], asum[PPT], tr[PPT], px[PPT], py[PPT];
   uint32_t walked[PPT];
 #pragma unroll
   for (int k = 0; k < PPT; ++k) {
@@ -354,7 +355,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     py[k] = float(iy) + 0.5f;
     cr[k] = cg[k] = cbl[k] = 0.f;
     asum[k] = (ix < img_w && iy < img_h) ? 0.f : 1.f;
-    tr[k] = tl[k] = 1.f;
+    tr[k] = 1.f;
     walked[k] = 0;
   }
   // The staging threads hold the next batch's records in registers, loaded
@@ -444,10 +445,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cr[k] = fmaf(q1.z, eo, cr[k]);
                 cg[k] = fmaf(q1.w, eo, cg[k]);
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
-                if constexpr (kState) {
-                  tl[k] = in ? tr[k] : tl[k];
-                  walked[k] = in ? wpos : walked[k];
-                }
+                if constexpr (kState) walked[k] = in ? wpos : walked[k];
                 tr[k] = tr[k] * (1.f - e);
                 asum[k] = asum[k] + eo;
               }
@@ -472,7 +470,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
     pix2rgb[3 * ip + 2] = cbl[k];
     if (pix2alpha) pix2alpha[ip] = asum[k];
     if (kState && pix2nwalk) pix2nwalk[ip] = walked[k];
-    if (kState && pix2tlast) pix2tlast[ip] = tl[k];
+    if (kState && pix2tfinal) pix2tfinal[ip] = tr[k];
   }
   if (kState && tile_cost) {
     // The tile's longest walk: what its adjoint (and the next frame's
@@ -623,7 +621,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
                                                 const float4* __restrict__ splat2,
                                                 const uint32_t* __restrict__ idx2pnt,
                                                 const uint32_t* __restrict__ pix2nwalk,
-                                                const float* __restrict__ pix2tlast,
+                                                const float* __restrict__ pix2tfinal,
                                                 const float* __restrict__ d_img,
                                                 const float* __restrict__ pix2rgb,
                                                 const float* __restrict__ target,
@@ -650,7 +648,10 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
   // only enter dL/de through this dot product
   float gb[PPT];
   // nw[k] = nwalk: the pixel's last contributing splat sits at position
-  // nw - 1, and its T is tlast itself (no division)
+  // nw - 1; tr starts at the pixel's final transmittance, behind that splat,
+  // and every splat of the walk back divides it by its own 1 - e (at the
+  // last contributor too: 1 - e is exact for e >= 1/2 and within half an ulp
+  // of 1 - e otherwise, so the recovered T is within ~1 ulp)
   uint32_t nw[PPT];
   uint32_t my_max = 0;
 #pragma unroll
@@ -662,7 +663,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
     if (ix < img_w && iy < img_h) {
       const size_t ip = size_t(iy) * img_w + ix;
       nw[k] = pix2nwalk[ip];
-      tr[k] = pix2tlast[ip];
+      tr[k] = pix2tfinal[ip];
       if (d_img) {
         g0[k] = d_img[3 * ip];
         g1[k] = d_img[3 * ip + 1];
@@ -778,7 +779,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           const float e = gk * q1.y;
           const float ome = 1.f - e;
           const float inv = __builtin_amdgcn_rcpf(ome);
-          tr[k] = it + 1u == nw[k] ? tr[k] : tr[k] * inv;
+          tr[k] = tr[k] * inv;
           const float t = tr[k];
           // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)
           float gc = fmaf(g0[k], q1.z, fmaf(g1[k], q1.w, g2[k] * q2.x));
@@ -873,7 +874,7 @@ __device__ __forceinline__ void raster_tile_bwd(uint32_t tx, uint32_t ty, uint32
                                                 const float4* __restrict__ splat2,
                                                 const uint32_t* __restrict__ idx2pnt,
                                                 const uint32_t* __restrict__ pix2nwalk,
-                                                const float* __restrict__ pix2tlast,
+                                                const float* __restrict__ pix2tfinal,
                                                 const float* __restrict__ d_img,
                                                 const float* __restrict__ pix2rgb,
                                                 const float* __restrict__ target,
@@ -882,11 +883,11 @@ __device__ __forceinline__ void raster_tile_bwd(uint32_t tx, uint32_t ty, uint32
   __shared__ BwdLds<TS, NT> lds;
   if (d_alpha)
     raster_tile_bwd_impl<TS, NT, true>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
-                                       pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
+                                       pix2tfinal, d_img, pix2rgb, target, d_alpha, gauss_base,
                                        out, lds);
   else
     raster_tile_bwd_impl<TS, NT, false>(tx, ty, img_w, img_h, end, splat2, idx2pnt, pix2nwalk,
-                                        pix2tlast, d_img, pix2rgb, target, d_alpha, gauss_base,
+                                        pix2tfinal, d_img, pix2rgb, target, d_alpha, gauss_base,
                                         out, lds);
 }
 

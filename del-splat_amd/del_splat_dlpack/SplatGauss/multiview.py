@@ -221,7 +221,7 @@ class MultiViewPlan:
                 self._copy(o[5], k, torch.int32))
 
     def state(self):
-        """(alpha, nwalk, tlast) of the last forward, concatenated per pixel
+        """(alpha, nwalk, tfinal) of the last forward, concatenated per pixel
         (alpha read from where that forward wrote it)."""
         o = self._outputs()
         if self._last_alpha is not None:

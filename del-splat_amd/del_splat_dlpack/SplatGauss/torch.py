@@ -166,27 +166,27 @@ def _check_grad_acc(grad_acc, n, dev):
 
 def rasterize_fwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, threads=0, tile_order=None,
                   tile_cost=None, state=True):
-    """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tlast [H,W]).
+    """Returns (rgb [H,W,3], alpha [H,W], nwalk [H,W] int32, tfinal [H,W]).
     tile_order (int32 [T], a permutation) schedules the tiles (results do not
     depend on it); tile_cost (int32 [T]) receives each tile's longest walk.
     state=False: a plain forward, without the adjoint's state (nwalk and
-    tlast are returned as None; the kernel skips tracking them)."""
+    tfinal are returned as None; the kernel skips tracking them)."""
     dev = splat2.device
     w, h = img_shape
     img = torch.empty((h, w, 3), device=dev, dtype=torch.float32)
     alpha = torch.empty((h, w), device=dev, dtype=torch.float32)
     nwalk = torch.empty((h, w), device=dev, dtype=torch.int32) if state else None
-    tlast = torch.empty((h, w), device=dev, dtype=torch.float32) if state else None
+    tfinal = torch.empty((h, w), device=dev, dtype=torch.float32) if state else None
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
     if tile_order is None and tile_cost is None:
         call("dsplat_gauss_rasterize_fwd", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
-             ptr(tile2idx), ptr(i2p), ptr(img), ptr(alpha), ptr(nwalk), ptr(tlast))
+             ptr(tile2idx), ptr(i2p), ptr(img), ptr(alpha), ptr(nwalk), ptr(tfinal))
     else:
         _check_schedule(tile2idx, tile_order, tile_cost)
         call("dsplat_gauss_rasterize_fwd_ordered", stream_ptr(dev), w, h, tile_size, threads,
              ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(img), ptr(alpha),
-             ptr(nwalk), ptr(tlast), ptr(tile_cost))
-    return img, alpha, nwalk, tlast
+             ptr(nwalk), ptr(tfinal), ptr(tile_cost))
+    return img, alpha, nwalk, tfinal
 
 
 def _check_schedule(tile2idx, *arrays):
@@ -207,7 +207,7 @@ def tile_order(tile_cost, out=None):
     return order
 
 
-def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, d_img,
+def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tfinal, d_img,
                   d_alpha=None, threads=0, tile_order=None, grad_acc=None):
     """Returns 2D gradients (d_pixxy [N,2], d_conic [N,3], d_opacity [N],
     d_rgb [N,3]). tile_order as for rasterize_fwd (tile_order(tile_cost) of
@@ -230,12 +230,12 @@ def rasterize_bwd(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast,
     da = None if d_alpha is None else d_alpha.contiguous()
     if tile_order is None:
         call("dsplat_gauss_rasterize_bwd", stream_ptr(dev), w, h, tile_size, threads,
-             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(nwalk), ptr(tlast),
+             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(nwalk), ptr(tfinal),
              ptr(d_img.contiguous()), ptr(da), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
     else:
         _check_schedule(tile2idx, tile_order)
         call("dsplat_gauss_rasterize_bwd_ordered", stream_ptr(dev), w, h, tile_size, threads,
-             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tlast),
+             ptr(splat2), ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tfinal),
              ptr(d_img.contiguous()), ptr(da), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
     return d_pixxy, d_conic, d_op, d_rgb
 
@@ -334,7 +334,7 @@ def sh_to_rgb(xyz, sh, cam_pos, degree=3):
     return _ShColour.apply(xyz, sh, _cam_pos(cam_pos, xyz.device), int(degree))
 
 
-def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tlast, img, target,
+def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tfinal, img, target,
                      threads=0, tile_order=None, grad_acc=None):
     """rasterize_bwd for the loss sum (img - target)^2: dL/drgb = 2 (img -
     target) is formed per pixel inside the adjoint kernel (img is the
@@ -357,7 +357,7 @@ def rasterize_bwd_l2(img_shape, tile_size, splat2, tile2idx, idx2pnt, nwalk, tla
     d_rgb = acc[6 * n:].view(n, 3)
     i2p = idx2pnt if idx2pnt.numel() else torch.zeros(1, device=dev, dtype=torch.int32)
     call("dsplat_gauss_rasterize_bwd_l2", stream_ptr(dev), w, h, tile_size, threads, ptr(splat2),
-         ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tlast), ptr(img.contiguous()),
+         ptr(tile2idx), ptr(i2p), ptr(tile_order), ptr(nwalk), ptr(tfinal), ptr(img.contiguous()),
          ptr(target.contiguous()), ptr(d_pixxy), ptr(d_conic), ptr(d_op), ptr(d_rgb))
     return d_pixxy, d_conic, d_op, d_rgb
 
@@ -386,12 +386,12 @@ class _Render(torch.autograd.Function):
         key = (str(xyz.device), tuple(img_shape), tile_size)
         hint = _SCHEDULE.get(key)
         cost = torch.empty(tile2idx.numel() - 1, device=xyz.device, dtype=torch.int32)
-        img, alpha, nwalk, tlast = rasterize_fwd(img_shape, tile_size, splat2, tile2idx,
+        img, alpha, nwalk, tfinal = rasterize_fwd(img_shape, tile_size, splat2, tile2idx,
                                                  idx2pnt, threads, tile_order=hint,
                                                  tile_cost=cost)
         order = tile_order(cost)
         _SCHEDULE[key] = order
-        ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast,
+        ctx.save_for_backward(xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tfinal,
                               order)
         ctx.gacc = gacc  # zeroed; handed to the first backward only
         ctx.img_shape = img_shape
@@ -402,13 +402,13 @@ class _Render(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, d_img, d_alpha):
-        (xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tlast,
+        (xyz, log_scale, quat, m16, splat2, tile2idx, idx2pnt, nwalk, tfinal,
          order) = ctx.saved_tensors
         if d_img is None:
             d_img = torch.zeros((ctx.img_shape[1], ctx.img_shape[0], 3), device=xyz.device)
         gacc, ctx.gacc = ctx.gacc, None
         d_pixxy, d_conic, d_op, d_rgb = rasterize_bwd(ctx.img_shape, ctx.tile_size, splat2,
-                                                      tile2idx, idx2pnt, nwalk, tlast, d_img,
+                                                      tile2idx, idx2pnt, nwalk, tfinal, d_img,
                                                       d_alpha, ctx.threads, tile_order=order,
                                                       grad_acc=gacc)
         d_xyz, d_ls, d_quat = project_bwd(xyz, log_scale, quat, m16, ctx.img_shape, d_pixxy,

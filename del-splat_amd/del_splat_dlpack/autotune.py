@@ -73,7 +73,7 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
         t2i, i2p = SG.bin_gauss(img_shape, ts, pxd, aabb)
         # timed with the walk-ordered tile schedule the render path uses
         cost = torch.empty(t2i.numel() - 1, device=t2i.device, dtype=torch.int32)
-        img, alpha, nwalk, tlast = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p,
+        img, alpha, nwalk, tfinal = SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p,
                                                     tile_cost=cost)
         order = SG.tile_order(cost)
         tf = {th: timed(lambda th=th: SG.rasterize_fwd(img_shape, ts, splat2, t2i, i2p, th,
@@ -82,7 +82,7 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
         thf = min(tf, key=tf.get)
         d_img = 2.0 * (img - target)
         tb = {th: timed(lambda th=th: SG.rasterize_bwd(img_shape, ts, splat2, t2i, i2p, nwalk,
-                                                      tlast, d_img, None, th, tile_order=order))
+                                                      tfinal, d_img, None, th, tile_order=order))
               for th in bwd_opts[ts]}
         thb = min(tb, key=tb.get)
         total = t_front + tf[thf] + tb[thb]
@@ -93,7 +93,7 @@ def tune_single(num_pnt, m16, img_shape, tensors, target, reps=2, candidates=Non
         res[f"{ts}/total"] = total
         if best is None or total < best[0]:
             best = (total, ts, thf, thb)
-        del pxd, conic, aabb, splat2, t2i, i2p, img, alpha, nwalk, tlast, d_img
+        del pxd, conic, aabb, splat2, t2i, i2p, img, alpha, nwalk, tfinal, d_img
     _, ts, thf, thb = best
     db[sig] = {"best": [ts, thf, thb], "ms": res}
     if use_cache:

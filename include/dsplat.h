@@ -201,8 +201,9 @@ int dsplat_gauss_project_bin_ws(void* stream, uint32_t num_pnt, const float* pnt
  * pix2rgb [H,W,3], pix2alpha [H,W] (the reference's alpha_sum), pix2nwalk
  * u32 [H,W] (1 + walk position of the last splat that contributed to the
  * pixel -- the terminating one if the walk terminated --, 0 if none),
- * pix2tlast [H,W] (transmittance in front of that splat, 1 if none).
- * pix2alpha / pix2nwalk / pix2tlast may be null. `threads` selects the
+ * pix2tfinal [H,W] (the pixel's transmittance after its walk, behind that
+ * splat; 1 if none).
+ * pix2alpha / pix2nwalk / pix2tfinal may be null. `threads` selects the
  * workgroup size per tile (0 = default; ts 16: 64/128/256, default 128;
  * ts 32: 128/256/512/1024, default 512; ts 8: always 64). Every size
  * renders bit-identical pixels. */
@@ -210,10 +211,10 @@ int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
                                uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                const uint32_t* tile2idx, const uint32_t* idx2pnt,
                                float* pix2rgb, float* pix2alpha, uint32_t* pix2nwalk,
-                               float* pix2tlast);
+                               float* pix2tfinal);
 
 /* Adjoint of dsplat_gauss_rasterize_fwd. Walks every pixel's list
- * back-to-front from pix2tlast. ADDS into d_pnt2pixxy [N,2], d_pnt2conic
+ * back-to-front from pix2tfinal. ADDS into d_pnt2pixxy [N,2], d_pnt2conic
  * [N,3], d_pnt2opacity [N], d_pnt2rgb [N,3]. d_pix2alpha may be null.
  * `threads` selects the workgroup size per tile (0 = default; ts 16:
  * 64/128/256, default 64; ts 32: 64/128/256/512/1024, default 256; ts 8:
@@ -221,7 +222,7 @@ int dsplat_gauss_rasterize_fwd(void* stream, uint32_t img_w, uint32_t img_h,
 int dsplat_gauss_rasterize_bwd(void* stream, uint32_t img_w, uint32_t img_h,
                                uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                const uint32_t* tile2idx, const uint32_t* idx2pnt,
-                               const uint32_t* pix2nwalk, const float* pix2tlast,
+                               const uint32_t* pix2nwalk, const float* pix2tfinal,
                                const float* d_pix2rgb, const float* d_pix2alpha,
                                float* d_pnt2pixxy, float* d_pnt2conic, float* d_pnt2opacity,
                                float* d_pnt2rgb);
@@ -241,12 +242,12 @@ int dsplat_gauss_rasterize_fwd_ordered(void* stream, uint32_t img_w, uint32_t im
                                        const float* pnt2splat2, const uint32_t* tile2idx,
                                        const uint32_t* idx2pnt, const uint32_t* tile_order,
                                        float* pix2rgb, float* pix2alpha, uint32_t* pix2nwalk,
-                                       float* pix2tlast, uint32_t* tile_cost);
+                                       float* pix2tfinal, uint32_t* tile_cost);
 int dsplat_gauss_rasterize_bwd_ordered(void* stream, uint32_t img_w, uint32_t img_h,
                                        uint32_t tile_size, uint32_t threads,
                                        const float* pnt2splat2, const uint32_t* tile2idx,
                                        const uint32_t* idx2pnt, const uint32_t* tile_order,
-                                       const uint32_t* pix2nwalk, const float* pix2tlast,
+                                       const uint32_t* pix2nwalk, const float* pix2tfinal,
                                        const float* d_pix2rgb, const float* d_pix2alpha,
                                        float* d_pnt2pixxy, float* d_pnt2conic,
                                        float* d_pnt2opacity, float* d_pnt2rgb);
@@ -262,7 +263,7 @@ int dsplat_gauss_rasterize_bwd_l2(void* stream, uint32_t img_w, uint32_t img_h,
                                   uint32_t tile_size, uint32_t threads, const float* pnt2splat2,
                                   const uint32_t* tile2idx, const uint32_t* idx2pnt,
                                   const uint32_t* tile_order, const uint32_t* pix2nwalk,
-                                  const float* pix2tlast, const float* pix2rgb,
+                                  const float* pix2tfinal, const float* pix2rgb,
                                   const float* target, float* d_pnt2pixxy, float* d_pnt2conic,
                                   float* d_pnt2opacity, float* d_pnt2rgb);
 
@@ -438,11 +439,11 @@ int dsplat_mv_plan_info(const dsplat_mv_plan* plan, uint32_t* num_view, uint64_t
 
 /* Device pointers of plan-owned state: alpha [pix] (written only by a
  * forward given a null out_alpha, and by the training step), nwalk u32 [pix],
- * tlast [pix], tile2idx u32 [tiles+1] (global tile ids), idx2pnt u32 [capacity]
+ * tfinal [pix], tile2idx u32 [tiles+1] (global tile ids), idx2pnt u32 [capacity]
  * (item = view*N + Gaussian), counters u32 [5] = (K, K if it fit else 0, -, -,
  * overflow flag). Any argument may be null. rgb is reserved (null). */
 int dsplat_mv_outputs(const dsplat_mv_plan* plan, float** rgb, float** alpha, uint32_t** nwalk,
-                      float** tlast, uint32_t** tile2idx, uint32_t** idx2pnt,
+                      float** tfinal, uint32_t** tile2idx, uint32_t** idx2pnt,
                       uint32_t** counters);
 
 /* Forward of all views: projection, binning + stable sort, persistent tiled

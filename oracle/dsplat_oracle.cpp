@@ -394,7 +394,8 @@ uint64_t tile2pnt_parallel(uint32_t n, const float* boxes, const float* pixxydep
 // Extra outputs beyond the reference's RGB: alpha (= the reference's
 // alpha_sum), nwalk (1 + list position of the last splat that contributed
 // to the pixel -- the terminating one if the walk terminated --, 0 if none)
-// and tlast (transmittance in front of that splat, 1 if none).
+// and tfinal (the pixel's transmittance after its walk, behind that splat; 1
+// if none).
 // If frozen_nwalk != null the walk length per pixel is taken from it and the
 // termination test is skipped (the function the adjoint differentiates).
 // exp_ulp != 0 moves every exp result that many ulps up (> 0) or down (< 0):
@@ -406,7 +407,7 @@ uint64_t tile2pnt_parallel(uint32_t n, const float* boxes, const float* pixxydep
 template <typename T>
 struct PixelOut {
   T c[3];
-  T alpha_sum, tlast;
+  T alpha_sum, tfinal;
   uint32_t walked;
 };
 
@@ -435,9 +436,9 @@ inline PixelOut<T> raster_pixel(uint32_t ix, uint32_t iy, const T* pixxydepth, c
     o.c[0] += rgb[3 * p + 0] * eo;
     o.c[1] += rgb[3 * p + 1] * eo;
     o.c[2] += rgb[3 * p + 2] * eo;
-    o.tlast = alpha_occu;
     o.walked = it + 1;
     alpha_occu *= T(1) - e;
+    o.tfinal = alpha_occu;
     o.alpha_sum += eo;
     // the reference compares the float sum against the double 0.999
     if (!frozen && double(o.alpha_sum) > 0.999) break;
@@ -450,7 +451,7 @@ template <typename T>
 void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const T* conic,
                      const float* aabb, const T* opacity, const T* rgb, uint32_t ts,
                      const uint32_t* tile2idx, const uint32_t* idx2pnt,
-                     T* pix2rgb, T* pix2alpha, uint32_t* pix2nwalk, T* pix2tlast,
+                     T* pix2rgb, T* pix2alpha, uint32_t* pix2nwalk, T* pix2tfinal,
                      const uint32_t* frozen_nwalk, int exp_ulp = 0) {
   const uint32_t tw = (img_w + ts - 1) / ts;
 #pragma omp parallel for schedule(dynamic, 2) if (g_threads > 1) num_threads(g_threads)
@@ -465,7 +466,7 @@ void gauss_rasterize(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const 
       for (int k = 0; k < 3; ++k) pix2rgb[3 * ipix + k] = o.c[k];
       if (pix2alpha) pix2alpha[ipix] = o.alpha_sum;
       if (pix2nwalk) pix2nwalk[ipix] = o.walked;
-      if (pix2tlast) pix2tlast[ipix] = o.tlast;
+      if (pix2tfinal) pix2tfinal[ipix] = o.tfinal;
     }
   }
 }
@@ -513,7 +514,9 @@ T gauss_l2_loss_tiles(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const
 // Adjoint of the tiled raster (new; structure follows the transfer
 // derivative of splat_gaussian2.rs:212-243 without its O(n^2) loop).
 // Walks each pixel's list back-to-front from the last contributing splat,
-// recovering the transmittance by division from tlast.
+// recovering the transmittance by division from tfinal: at every
+// contributing splat, the last one included, T in front of it is T behind it
+// divided by its 1 - e.
 // Gradients accumulate (+=) into: d_pixxy[N,2], d_conic[N,3], d_opacity[N],
 // d_rgb[N,3]. d_pix2alpha may be null.
 // ---------------------------------------------------------------------------
@@ -521,7 +524,7 @@ template <typename T>
 void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, const T* conic,
                          const float* aabb, const T* opacity, const T* rgb, uint32_t ts,
                          const uint32_t* tile2idx, const uint32_t* idx2pnt,
-                         const uint32_t* pix2nwalk, const T* pix2tlast,
+                         const uint32_t* pix2nwalk, const T* pix2tfinal,
                          const T* d_pix2rgb, const T* d_pix2alpha,
                          T* d_pixxy, T* d_conic, T* d_opacity, T* d_rgb,
                          uint32_t row0 = 0, uint32_t row1 = ~0u) {
@@ -534,8 +537,7 @@ void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, co
       const uint32_t i0 = tile2idx[tile], num = tile2idx[tile + 1] - i0;
       const T g[3] = {d_pix2rgb[3 * ipix], d_pix2rgb[3 * ipix + 1], d_pix2rgb[3 * ipix + 2]};
       const T ga = d_pix2alpha ? d_pix2alpha[ipix] : T(0);
-      T trans = pix2tlast[ipix];  // transmittance in front of the current splat
-      bool first = true;
+      T trans = pix2tfinal[ipix];  // transmittance behind the current splat
       T back[3] = {0, 0, 0};      // sum_{m behind} c_m e_m T_m
       T back_a = 0;               // sum_{m behind} e_m T_m
       for (int it = int(pix2nwalk[ipix]) - 1; it >= 0; --it) {
@@ -549,8 +551,7 @@ void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, co
         const T gk = std::exp(T(-0.5) * q);
         const T op = opacity[p];
         const T e = gk * op;
-        if (!first) trans = trans / (T(1) - e);
-        first = false;
+        trans = trans / (T(1) - e);  // now in front of it
         const T one_m_e = T(1) - e;
         // dL/de
         T dl_de = 0;
@@ -581,7 +582,7 @@ template <typename T>
 void gauss_rasterize_bwd_mt(uint32_t n, uint32_t img_w, uint32_t img_h, const T* pixxydepth,
                             const T* conic, const float* aabb, const T* opacity, const T* rgb,
                             uint32_t ts, const uint32_t* tile2idx, const uint32_t* idx2pnt,
-                            const uint32_t* pix2nwalk, const T* pix2tlast, const T* d_pix2rgb,
+                            const uint32_t* pix2nwalk, const T* pix2tfinal, const T* d_pix2rgb,
                             const T* d_pix2alpha, T* d_pixxy, T* d_conic, T* d_opacity,
                             T* d_rgb) {
   const int nthr = g_threads;
@@ -594,7 +595,7 @@ void gauss_rasterize_bwd_mt(uint32_t n, uint32_t img_w, uint32_t img_h, const T*
     // on every run; interleaved, as the image centre is the heavy part)
     for (uint32_t r0 = 4u * uint32_t(t); r0 < img_h; r0 += 4u * uint32_t(nthr))
       gauss_rasterize_bwd<T>(img_w, img_h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
-                             idx2pnt, pix2nwalk, pix2tlast, d_pix2rgb, d_pix2alpha, b, b + 2 * n,
+                             idx2pnt, pix2nwalk, pix2tfinal, d_pix2rgb, d_pix2alpha, b, b + 2 * n,
                              b + 5 * n, b + 6 * n, r0, r0 + 4u);
   }
 #pragma omp parallel for schedule(static) num_threads(nthr)
@@ -1132,17 +1133,17 @@ void orc_gauss_rasterize_f32(uint32_t w, uint32_t h, const float* pixxydepth, co
                              const float* aabb, const float* opacity, const float* rgb,
                              uint32_t ts, const uint32_t* tile2idx, const uint32_t* idx2pnt,
                              float* pix2rgb, float* pix2alpha, uint32_t* pix2nwalk,
-                             float* pix2tlast, const uint32_t* frozen) {
+                             float* pix2tfinal, const uint32_t* frozen) {
   gauss_rasterize<float>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                         pix2rgb, pix2alpha, pix2nwalk, pix2tlast, frozen);
+                         pix2rgb, pix2alpha, pix2nwalk, pix2tfinal, frozen);
 }
 void orc_gauss_rasterize_exp_ulp_f32(uint32_t w, uint32_t h, const float* pixxydepth,
                                      const float* conic, const float* aabb, const float* opacity,
                                      const float* rgb, uint32_t ts, const uint32_t* tile2idx,
                                      const uint32_t* idx2pnt, float* pix2rgb, float* pix2alpha,
-                                     uint32_t* pix2nwalk, float* pix2tlast, int exp_ulp) {
+                                     uint32_t* pix2nwalk, float* pix2tfinal, int exp_ulp) {
   gauss_rasterize<float>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                         pix2rgb, pix2alpha, pix2nwalk, pix2tlast, nullptr, exp_ulp);
+                         pix2rgb, pix2alpha, pix2nwalk, pix2tfinal, nullptr, exp_ulp);
 }
 double orc_gauss_l2_loss_tiles_f64(uint32_t w, uint32_t h, const double* pixxydepth,
                                    const double* conic, const float* aabb, const double* opacity,
@@ -1157,20 +1158,20 @@ void orc_gauss_rasterize_f64(uint32_t w, uint32_t h, const double* pixxydepth,
                              const double* conic, const float* aabb, const double* opacity,
                              const double* rgb, uint32_t ts, const uint32_t* tile2idx,
                              const uint32_t* idx2pnt, double* pix2rgb, double* pix2alpha,
-                             uint32_t* pix2nwalk, double* pix2tlast, const uint32_t* frozen) {
+                             uint32_t* pix2nwalk, double* pix2tfinal, const uint32_t* frozen) {
   gauss_rasterize<double>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                          pix2rgb, pix2alpha, pix2nwalk, pix2tlast, frozen);
+                          pix2rgb, pix2alpha, pix2nwalk, pix2tfinal, frozen);
 }
 
 void orc_gauss_rasterize_bwd_f32(uint32_t w, uint32_t h, const float* pixxydepth,
                                  const float* conic, const float* aabb, const float* opacity,
                                  const float* rgb, uint32_t ts, const uint32_t* tile2idx,
                                  const uint32_t* idx2pnt, const uint32_t* nwalk,
-                                 const float* tlast, const float* d_rgbimg,
+                                 const float* tfinal, const float* d_rgbimg,
                                  const float* d_alphaimg, float* d_pixxy, float* d_conic,
                                  float* d_opacity, float* d_rgb) {
   gauss_rasterize_bwd<float>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                             nwalk, tlast, d_rgbimg, d_alphaimg, d_pixxy, d_conic, d_opacity,
+                             nwalk, tfinal, d_rgbimg, d_alphaimg, d_pixxy, d_conic, d_opacity,
                              d_rgb);
 }
 // Threads of the CPU loops (1 = the sequential parity oracle); returns the
@@ -1185,11 +1186,11 @@ void orc_gauss_rasterize_bwd_mt_f32(uint32_t n, uint32_t w, uint32_t h, const fl
                                     const float* conic, const float* aabb, const float* opacity,
                                     const float* rgb, uint32_t ts, const uint32_t* tile2idx,
                                     const uint32_t* idx2pnt, const uint32_t* nwalk,
-                                    const float* tlast, const float* d_rgbimg,
+                                    const float* tfinal, const float* d_rgbimg,
                                     const float* d_alphaimg, float* d_pixxy, float* d_conic,
                                     float* d_opacity, float* d_rgb) {
   gauss_rasterize_bwd_mt<float>(n, w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
-                                idx2pnt, nwalk, tlast, d_rgbimg, d_alphaimg, d_pixxy, d_conic,
+                                idx2pnt, nwalk, tfinal, d_rgbimg, d_alphaimg, d_pixxy, d_conic,
                                 d_opacity, d_rgb);
 }
 
@@ -1197,11 +1198,11 @@ void orc_gauss_rasterize_bwd_f64(uint32_t w, uint32_t h, const double* pixxydept
                                  const double* conic, const float* aabb, const double* opacity,
                                  const double* rgb, uint32_t ts, const uint32_t* tile2idx,
                                  const uint32_t* idx2pnt, const uint32_t* nwalk,
-                                 const double* tlast, const double* d_rgbimg,
+                                 const double* tfinal, const double* d_rgbimg,
                                  const double* d_alphaimg, double* d_pixxy, double* d_conic,
                                  double* d_opacity, double* d_rgb) {
   gauss_rasterize_bwd<double>(w, h, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                              nwalk, tlast, d_rgbimg, d_alphaimg, d_pixxy, d_conic, d_opacity,
+                              nwalk, tfinal, d_rgbimg, d_alphaimg, d_pixxy, d_conic, d_opacity,
                               d_rgb);
 }
 

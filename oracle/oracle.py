@@ -209,7 +209,7 @@ def tile_rects(boxes, ts, tw, th):
 # -------------------------------------------------------------------- raster
 def gauss_rasterize(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
                     dtype=np.float32, frozen_nwalk=None):
-    """splat_gauss.cu:56-109 plus alpha / nwalk / tlast."""
+    """splat_gauss.cu:56-109 plus alpha / nwalk / tfinal."""
     w, h = img_shape
     pxd, cn, op, col = (_c(a, dtype).reshape(-1) for a in (pixxydepth, conic, opacity, rgb))
     bb = _c(aabb, np.float32).reshape(-1)
@@ -220,11 +220,11 @@ def gauss_rasterize(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2i
     img = np.zeros(h * w * 3, dtype)
     alpha = np.zeros(h * w, dtype)
     nwalk = np.zeros(h * w, np.uint32)
-    tlast = np.zeros(h * w, dtype)
+    tfinal = np.zeros(h * w, dtype)
     fz = None if frozen_nwalk is None else _c(frozen_nwalk, np.uint32).reshape(-1)
     f = lib().orc_gauss_rasterize_f32 if dtype == np.float32 else lib().orc_gauss_rasterize_f64
-    f(w, h, pxd, cn, bb, op, col, ts, t2i, i2p, img, alpha, nwalk, tlast, _ptr(fz))
-    return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tlast.reshape(h, w))
+    f(w, h, pxd, cn, bb, op, col, ts, t2i, i2p, img, alpha, nwalk, tfinal, _ptr(fz))
+    return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tfinal.reshape(h, w))
 
 
 def gauss_rasterize_exp_ulp(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
@@ -240,11 +240,11 @@ def gauss_rasterize_exp_ulp(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts
     img = np.zeros(h * w * 3, np.float32)
     alpha = np.zeros(h * w, np.float32)
     nwalk = np.zeros(h * w, np.uint32)
-    tlast = np.zeros(h * w, np.float32)
+    tfinal = np.zeros(h * w, np.float32)
     lib().orc_gauss_rasterize_exp_ulp_f32(w, h, pxd, cn, bb, op, col, ts,
-                                          _c(tile2idx, np.uint32), i2p, img, alpha, nwalk, tlast,
+                                          _c(tile2idx, np.uint32), i2p, img, alpha, nwalk, tfinal,
                                           int(exp_ulp))
-    return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tlast.reshape(h, w))
+    return (img.reshape(h, w, 3), alpha.reshape(h, w), nwalk.reshape(h, w), tfinal.reshape(h, w))
 
 
 def gauss_l2_loss_tiles(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
@@ -281,10 +281,10 @@ def self_flips(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, i
 
 
 def gauss_rasterize_bwd(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx, idx2pnt,
-                        nwalk, tlast, d_img, d_alpha=None, dtype=np.float32):
+                        nwalk, tfinal, d_img, d_alpha=None, dtype=np.float32):
     w, h = img_shape
     pxd, cn, op, col, tl, dimg = (_c(a, dtype).reshape(-1)
-                                  for a in (pixxydepth, conic, opacity, rgb, tlast, d_img))
+                                  for a in (pixxydepth, conic, opacity, rgb, tfinal, d_img))
     bb = _c(aabb, np.float32).reshape(-1)
     n = op.size
     da = None if d_alpha is None else _c(d_alpha, dtype).reshape(-1)
@@ -310,12 +310,12 @@ def set_threads(n):
 
 
 def gauss_rasterize_bwd_mt(img_shape, pixxydepth, conic, aabb, opacity, rgb, ts, tile2idx,
-                           idx2pnt, nwalk, tlast, d_img, d_alpha=None):
+                           idx2pnt, nwalk, tfinal, d_img, d_alpha=None):
     """gauss_rasterize_bwd (float) over the threads set by set_threads (row
     bands into per-thread gradient buffers)."""
     w, h = img_shape
     pxd, cn, op, col, tl, dimg = (_c(a, np.float32).reshape(-1)
-                                  for a in (pixxydepth, conic, opacity, rgb, tlast, d_img))
+                                  for a in (pixxydepth, conic, opacity, rgb, tfinal, d_img))
     bb = _c(aabb, np.float32).reshape(-1)
     n = op.size
     da = None if d_alpha is None else _c(d_alpha, np.float32).reshape(-1)

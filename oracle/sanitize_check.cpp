@@ -146,17 +146,17 @@ int main() {
   std::vector<uint32_t> i2p(K + 1);
   orc_tile2pnt(n, aabb.data(), pxd.data(), w, h, ts, t2i.data(), i2p.data(), nullptr);
   const size_t P = size_t(w) * h;
-  std::vector<float> img(3 * P), alpha(P), tlast(P), target(3 * P);
+  std::vector<float> img(3 * P), alpha(P), tfinal(P), target(3 * P);
   std::vector<uint32_t> nwalk(P);
   orc_gauss_rasterize_f32(w, h, pxd.data(), conic.data(), aabb.data(), op.data(), rgb.data(), ts,
                           t2i.data(), i2p.data(), img.data(), alpha.data(), nwalk.data(),
-                          tlast.data(), nullptr);
+                          tfinal.data(), nullptr);
   orc_gauss_rasterize_exp_ulp_f32(w, h, pxd.data(), conic.data(), aabb.data(), op.data(),
                                   rgb.data(), ts, t2i.data(), i2p.data(), img.data(),
-                                  alpha.data(), nwalk.data(), tlast.data(), 1);
+                                  alpha.data(), nwalk.data(), tfinal.data(), 1);
   orc_gauss_rasterize_f32(w, h, pxd.data(), conic.data(), aabb.data(), op.data(), rgb.data(), ts,
                           t2i.data(), i2p.data(), img.data(), alpha.data(), nwalk.data(),
-                          tlast.data(), nullptr);
+                          tfinal.data(), nullptr);
   {
     std::vector<double> p64(pxd.begin(), pxd.end()), c64(conic.begin(), conic.end()),
         o64(op.begin(), op.end()), r64(rgb.begin(), rgb.end()), img64(3 * P), a64(P), t64(P);
@@ -168,7 +168,7 @@ int main() {
   for (size_t k = 0; k < 3 * P; ++k) target[k] = 2.f * (img[k] - uni(0.f, 1.f));
   std::vector<float> dpix(2 * n), dcon(3 * n), dop(n), drgb(3 * n), dalpha(P, 0.25f);
   orc_gauss_rasterize_bwd_f32(w, h, pxd.data(), conic.data(), aabb.data(), op.data(), rgb.data(),
-                              ts, t2i.data(), i2p.data(), nwalk.data(), tlast.data(),
+                              ts, t2i.data(), i2p.data(), nwalk.data(), tfinal.data(),
                               target.data(), dalpha.data(), dpix.data(), dcon.data(), dop.data(),
                               drgb.data());
   const int nth = orc_set_threads(4);
@@ -176,7 +176,7 @@ int main() {
   if (nth < 1) return fail("set_threads");
   orc_gauss_rasterize_bwd_mt_f32(n, w, h, pxd.data(), conic.data(), aabb.data(),
                                  op.data(), rgb.data(), ts, t2i.data(), i2p.data(), nwalk.data(),
-                                 tlast.data(), target.data(), nullptr, dpix2.data(), dcon2.data(),
+                                 tfinal.data(), target.data(), nullptr, dpix2.data(), dcon2.data(),
                                  dop2.data(), drgb2.data());
   orc_set_threads(1);
   std::vector<float> dxyz(3 * n), dsc(3 * n), dq(4 * n);

@@ -71,9 +71,9 @@ def test_bulba_full_chain(oracle, view, ts, th_bwd):
     pxd, conic, aabb, splat2 = SG.project_fwd(xyz, sc, q, op, rgb, m16, SHAPE, log_scale=False)
     t2i, i2p = SG.tile2idx_idx2pnt(SHAPE, ts, pxd, aabb)
     cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(SHAPE, ts, splat2, t2i, i2p, tile_cost=cost)
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(SHAPE, ts, splat2, t2i, i2p, tile_cost=cost)
     order = SG.tile_order(cost)
-    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(SHAPE, ts, splat2, t2i, i2p, nwalk, tlast, img,
+    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(SHAPE, ts, splat2, t2i, i2p, nwalk, tfinal, img,
                                                 _t(target), th_bwd, tile_order=order)
     dxyz, dsc, dq = SG.project_bwd(xyz, sc, q, m16, SHAPE, dpix, dcon, log_scale=False)
     torch.cuda.synchronize()
@@ -95,7 +95,7 @@ def test_bulba_full_chain(oracle, view, ts, th_bwd):
     np.testing.assert_array_equal(u32(i2p), c_i2p)
     print(f"pairs {len(o_i2p)}", flush=True)
     # raster
-    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(
         SHAPE, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
     own = oracle.self_flips(SHAPE, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i,
                             o_i2p, o_nwalk)
@@ -104,7 +104,7 @@ def test_bulba_full_chain(oracle, view, ts, th_bwd):
     assert (o_alpha >= 0.999).mean() > 0.05 and (o_alpha > 0.01).mean() > 0.2
     # adjoints
     o_d = oracle.gauss_rasterize_bwd(SHAPE, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts,
-                                     o_t2i, o_i2p, o_nwalk, o_tlast,
+                                     o_t2i, o_i2p, o_nwalk, o_tfinal,
                                      (2.0 * (o_img - target)).astype(np.float32))
     o_dxyz, o_dsc, o_dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, SHAPE,
                                                    o_d[0], o_d[1])

@@ -38,8 +38,8 @@ def test_hip_gradients_match_finite_differences_headline(oracle):
     pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, m16, shape)
     t2i, i2p = SG.bin_gauss(shape, ts, pxd, aabb)
     cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, 512, tile_cost=cost)
-    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast, img,
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, 512, tile_cost=cost)
+    dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tfinal, img,
                                                 _t(target), 256, tile_order=SG.tile_order(cost))
     dxyz, dls, dq = SG.project_bwd(xyz, ls, q, m16, shape, dpix, dcon)
     torch.cuda.synchronize()

@@ -297,14 +297,14 @@ def test_raster_parity(oracle, ts):
     t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
     op, rgb = _t(s["opacity"]), _t(s["rgb"])
     img = SG.rasterize_pnt2splat2(shape, pxd, conic, aabb, op, rgb, ts, t2i, i2p).cpu().numpy()
-    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(
         shape, pxd.cpu().numpy(), conic.cpu().numpy(), aabb.cpu().numpy(), s["opacity"],
         s["rgb"], ts, u32(t2i), u32(i2p))
     # packed-record path with the adjoint's state
     _, _, _, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op, rgb,
                                      _t(m), shape, log_scale=False)
     for thr in ([0, 64, 128, 256] if ts == 16 else [0, 128, 256, 512, 1024] if ts == 32 else [0]):
-        g_img, g_alpha, g_nwalk, g_tlast = (t.cpu().numpy() for t in SG.rasterize_fwd(
+        g_img, g_alpha, g_nwalk, g_tfinal = (t.cpu().numpy() for t in SG.rasterize_fwd(
             shape, ts, splat2, t2i, i2p, thr))
         flips = _check_pixels(g_img, g_alpha, g_nwalk.view(np.uint32), o_img, o_alpha, o_nwalk)
         print(f"ts={ts} threads={thr} threshold flips={flips}")
@@ -352,12 +352,12 @@ def _grad_case(oracle, n, shape, seed, ts, with_alpha):
     scale = np.exp(s["log_scale"]).astype(np.float32)
     o_pxd, o_conic, o_aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], scale, m, shape)
     o_t2i, o_i2p = oracle.tile2pnt_gauss(o_pxd, o_aabb, shape, ts)
-    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
     d_img = 2.0 * (img.detach().cpu().numpy() - target)
     dpix, dcon, dop, drgb = oracle.gauss_rasterize_bwd(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p, o_nwalk,
-        o_tlast, d_img, w_alpha)
+        o_tfinal, d_img, w_alpha)
     dxyz, dls, dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], scale, m, shape, dpix, dcon,
                                              log_scale=True)
     return dict(xyz=(xyz.grad, dxyz), log_scale=(ls.grad, dls), quat=(q.grad, dq),
@@ -385,16 +385,16 @@ def test_raster_bwd_every_workgroup_shape(oracle, ts, threads, with_alpha):
     pxd, conic, aabb, splat2 = SG.project_fwd(_t(s["xyz"]), _t(s["scale"]), _t(s["quat"]), op,
                                               rgb, _t(m), shape, log_scale=False)
     t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p)
     rng = np.random.default_rng(ts + threads)
     d_img = rng.standard_normal((shape[1], shape[0], 3)).astype(np.float32)
     d_alpha = rng.standard_normal((shape[1], shape[0])).astype(np.float32) if with_alpha else None
-    g = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast, _t(d_img),
+    g = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tfinal, _t(d_img),
                          None if d_alpha is None else _t(d_alpha), threads)
     o = oracle.gauss_rasterize_bwd(shape, pxd.cpu().numpy(), conic.cpu().numpy(),
                                    aabb.cpu().numpy(), s["opacity"], s["rgb"], ts, u32(t2i),
                                    u32(i2p), nwalk.cpu().numpy().view(np.uint32),
-                                   tlast.cpu().numpy(), d_img, d_alpha)
+                                   tfinal.cpu().numpy(), d_img, d_alpha)
     for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
         check_grad(a.cpu().numpy(), b, f"ts={ts} threads={threads} alpha={with_alpha} {name}",
                    abs_frac=1e-4)
@@ -412,13 +412,13 @@ def test_raster_bwd_fused_l2(ts, threads):
     pxd, conic, aabb, splat2 = SG.project_fwd(xyz, ls, q, op, rgb, _t(m), shape)
     t2i, i2p = SG.tile2idx_idx2pnt(shape, ts, pxd, aabb)
     cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, threads,
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, threads,
                                                 tile_cost=cost)
     order = SG.tile_order(cost)
     target = _t(scene.target_image(shape, 77))
-    ref = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tlast, 2.0 * (img - target), None,
+    ref = SG.rasterize_bwd(shape, ts, splat2, t2i, i2p, nwalk, tfinal, 2.0 * (img - target), None,
                            threads, tile_order=order)
-    got = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast, img, target, threads,
+    got = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tfinal, img, target, threads,
                               tile_order=order)
     for a, b in zip(got, ref):
         # same terms, float atomics in another order
@@ -496,10 +496,10 @@ def test_project_fwd_clears_accumulators_record_only(n):
     np.testing.assert_array_equal(splat2.cpu().numpy()[:, 2:5], conic.cpu().numpy())
     assert (gacc[:9 * n] == 0).all() and gacc[9 * n:].isnan().all()
     t2i, i2p = SG.tile2idx_idx2pnt(shape, 16, pxd, aabb)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, 16, splat2, t2i, i2p)
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, 16, splat2, t2i, i2p)
     target = _t(scene.target_image(shape, 78))
-    ref = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tlast, img, target)
-    got = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tlast, img, target,
+    ref = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tfinal, img, target)
+    got = SG.rasterize_bwd_l2(shape, 16, splat2, t2i, i2p, nwalk, tfinal, img, target,
                               grad_acc=gacc[:9 * n])
     for a, b in zip(got, ref):
         d = (a - b).abs().max().item()
@@ -598,11 +598,11 @@ def test_headline_frame_matches_oracle(oracle):
     pxd, conic, aabb, splat2 = SG.project_fwd(xyz, sc, q, op, rgb, m16, shape, log_scale=False)
     t2i, i2p = SG.bin_gauss(shape, ts, pxd, aabb)
     cost = torch.empty(t2i.numel() - 1, device=DEV, dtype=torch.int32)
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, tile_cost=cost)
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, t2i, i2p, th, tile_cost=cost)
     order = SG.tile_order(cost)
     grads = {}
     for thb in (256, 512):
-        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tlast,
+        dpix, dcon, dop, drgb = SG.rasterize_bwd_l2(shape, ts, splat2, t2i, i2p, nwalk, tfinal,
                                                     img, _t(target), thb, tile_order=order)
         dxyz, dsc, dq = SG.project_bwd(xyz, sc, q, m16, shape, dpix, dcon, log_scale=False)
         grads[thb] = [t.cpu().numpy() for t in (dxyz, dsc, dq, dop, drgb)]
@@ -614,7 +614,7 @@ def test_headline_frame_matches_oracle(oracle):
     o_t2i, o_i2p = oracle.tile2pnt_gauss(o_pxd, o_aabb, shape, ts)
     np.testing.assert_array_equal(u32(t2i), o_t2i)
     np.testing.assert_array_equal(u32(i2p), o_i2p)
-    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p)
     own = oracle.self_flips(shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i,
                             o_i2p, o_nwalk)
@@ -624,7 +624,7 @@ def test_headline_frame_matches_oracle(oracle):
     print(f"threshold flips: {flips}")
     o_dpix, o_dcon, o_dop, o_drgb = oracle.gauss_rasterize_bwd(
         shape, o_pxd, o_conic, o_aabb, s["opacity"], s["rgb"], ts, o_t2i, o_i2p, o_nwalk,
-        o_tlast, (2.0 * (o_img - target)).astype(np.float32))
+        o_tfinal, (2.0 * (o_img - target)).astype(np.float32))
     o_dxyz, o_dsc, o_dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape,
                                                    o_dpix, o_dcon)
     for thb, gl in grads.items():
@@ -694,17 +694,17 @@ def test_box_edges_on_pixel_centres(oracle, ts, threads):
     order = np.argsort(pxd[:, 2], kind="stable").astype(np.uint32)
     t2i = (np.arange(ntile + 1) * n).astype(np.uint32)
     i2p = np.tile(order, ntile)
-    o_img, o_alpha, o_nwalk, o_tlast = oracle.gauss_rasterize(shape, pxd, conic, box, op, rgb,
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(shape, pxd, conic, box, op, rgb,
                                                               ts, t2i, i2p)
     g_t2i, g_i2p = _t(t2i.view(np.int32)), _t(i2p.view(np.int32))
-    img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, _t(splat2), g_t2i, g_i2p, threads)
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, _t(splat2), g_t2i, g_i2p, threads)
     _check_pixels(img.cpu().numpy(), alpha.cpu().numpy(), nwalk.cpu().numpy().view(np.uint32),
                   o_img, o_alpha, o_nwalk)
     d_img = np.random.default_rng(ts).standard_normal((shape[1], shape[0], 3)).astype(np.float32)
-    g = SG.rasterize_bwd(shape, ts, _t(splat2), g_t2i, g_i2p, nwalk, tlast, _t(d_img), None,
+    g = SG.rasterize_bwd(shape, ts, _t(splat2), g_t2i, g_i2p, nwalk, tfinal, _t(d_img), None,
                          threads)
     o = oracle.gauss_rasterize_bwd(shape, pxd, conic, box, op, rgb, ts, t2i, i2p,
-                                   nwalk.cpu().numpy().view(np.uint32), tlast.cpu().numpy(),
+                                   nwalk.cpu().numpy().view(np.uint32), tfinal.cpu().numpy(),
                                    d_img)
     for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
         check_grad(a.cpu().numpy(), b, f"ts={ts} threads={threads} {name}", abs_frac=1e-4)

@@ -40,7 +40,7 @@ def test_multiview_forward_matches_per_view(oracle, ts, threads):
     plan = MultiViewPlan(n, views, tile_size=ts, threads=threads)
     out_rgb, out_alpha = plan.forward(xyz, ls, q, op, rgb)
     t2i, i2p = plan.lists()
-    alpha_all, nwalk_all, tlast_all = plan.state()
+    alpha_all, nwalk_all, tfinal_all = plan.state()
     g_t2i = u32(t2i).astype(np.int64)
     g_i2p = u32(i2p).astype(np.int64)
     imgs = plan.split(out_rgb, 3)
@@ -54,7 +54,7 @@ def test_multiview_forward_matches_per_view(oracle, ts, threads):
         np.testing.assert_array_equal(loc_i2p, u32(s_i2p))
         o_t2i, o_i2p = oracle.tile2pnt_gauss(pxd.cpu().numpy(), aabb.cpu().numpy(), shape, ts)
         np.testing.assert_array_equal(loc_i2p, o_i2p)
-        img, alpha, nwalk, tlast = SG.rasterize_fwd(shape, ts, splat2, s_t2i, s_i2p, threads)
+        img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, splat2, s_t2i, s_i2p, threads)
         np.testing.assert_array_equal(imgs[v].cpu().numpy(), img.cpu().numpy())
         pa, pb = plan.pix_off[v], plan.pix_off[v + 1]
         np.testing.assert_array_equal(nwalk_all[pa:pb].cpu().numpy(), nwalk.reshape(-1).cpu().numpy())

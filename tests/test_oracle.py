@@ -240,12 +240,12 @@ def test_single_and_stacked_splats_closed_form(oracle):
     op = np.array([0.6, 0.5], np.float32)
     rgb = np.array([[1, 0, 0], [0, 1, 0]], np.float32)
     t2i, i2p = oracle.tile2pnt_gauss(pxd, aabb, shape, 16)
-    img, alpha, nwalk, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb, op, rgb, 16, t2i,
+    img, alpha, nwalk, tfinal = oracle.gauss_rasterize(shape, pxd, conic, aabb, op, rgb, 16, t2i,
                                                       i2p)
     # pixel centre (8.5, 8.5) is both means: e = opacity
     np.testing.assert_allclose(img[8, 8], [0.6, 0.5 * 0.4, 0.0], rtol=1e-6)
     np.testing.assert_allclose(alpha[8, 8], 0.6 + 0.5 * 0.4, rtol=1e-6)
-    np.testing.assert_allclose(tlast[8, 8], 0.4, rtol=1e-6)
+    np.testing.assert_allclose(tfinal[8, 8], 0.4 * 0.5, rtol=1e-6)  # after both splats
     # one pixel to the right: q = 0.1 * 1^2
     e = np.exp(-0.05)
     np.testing.assert_allclose(img[8, 9], [0.6 * e, 0.5 * e * (1 - 0.6 * e), 0], rtol=1e-6)
@@ -414,10 +414,10 @@ def test_end_to_end_fd_64_gaussians(oracle):
     def loss(P):
         return ((fwd(P)[2][0] - target) ** 2).sum()
 
-    pxd, conic, (img, alpha, _, tlast) = fwd(P0)
+    pxd, conic, (img, alpha, _, tfinal) = fwd(P0)
     d_img = 2 * (img - target)
     dpix, dcon, dop, drgb = oracle.gauss_rasterize_bwd(shape, pxd, conic, aabb, P0["op"],
-                                                       P0["rgb"], ts, t2i, i2p, nwalk, tlast,
+                                                       P0["rgb"], ts, t2i, i2p, nwalk, tfinal,
                                                        d_img, dtype=f64)
     dxyz, dls, dq = oracle.gauss_project_bwd(P0["xyz"], P0["q"], np.exp(P0["ls"]), m64, shape,
                                              dpix, dcon, log_scale=True, dtype=f64)
@@ -608,12 +608,12 @@ def test_exp_ulp_perturbation_and_self_flips(oracle):
     s, m = small_scene(20000, 70, shape)
     pxd, conic, aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
     t2i, i2p = oracle.tile2pnt_gauss(pxd, aabb, shape, ts)
-    img, alpha, nwalk, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"],
+    img, alpha, nwalk, tfinal = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"],
                                                       s["rgb"], ts, t2i, i2p)
     assert (alpha >= 0.999).mean() > 0.05
     z = oracle.gauss_rasterize_exp_ulp(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i,
                                        i2p, 0)
-    for a, b in zip(z, (img, alpha, nwalk, tlast)):
+    for a, b in zip(z, (img, alpha, nwalk, tfinal)):
         np.testing.assert_array_equal(a, b)
     for u in (1, -1, 3):
         img2, alpha2, nw2, _ = oracle.gauss_rasterize_exp_ulp(shape, pxd, conic, aabb,
@@ -647,7 +647,7 @@ def test_tile_loss_and_local_fd_match_the_adjoint(oracle):
     pxd, conic, _, _ = oracle.gauss_project(s["xyz"].astype(f64), s["quat"].astype(f64),
                                             np.exp(s["log_scale"].astype(f64)), m.astype(f64),
                                             shape, dtype=f64)
-    img, alpha, _, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"], s["rgb"],
+    img, alpha, _, tfinal = oracle.gauss_rasterize(shape, pxd, conic, aabb, s["opacity"], s["rgb"],
                                                   ts, t2i, i2p, dtype=f64, frozen_nwalk=nwalk)
     full = ((img - target.astype(f64)) ** 2).sum()
     tw, th = (shape[0] + ts - 1) // ts, (shape[1] + ts - 1) // ts
@@ -657,7 +657,7 @@ def test_tile_loss_and_local_fd_match_the_adjoint(oracle):
                                        i2p, nwalk.astype(np.uint32), target, (0, tw - 1, 0, th - 1))
     assert abs(tiles - full) <= 1e-12 * full
     dpix, dcon, dop, drgb = oracle.gauss_rasterize_bwd(shape, pxd, conic, aabb, s["opacity"],
-                                                       s["rgb"], ts, t2i, i2p, nwalk, tlast,
+                                                       s["rgb"], ts, t2i, i2p, nwalk, tfinal,
                                                        2 * (img - target), dtype=f64)
     dxyz, dls, dq = oracle.gauss_project_bwd(s["xyz"], s["quat"], np.exp(s["log_scale"]), m,
                                              shape, dpix, dcon, log_scale=True, dtype=f64)
@@ -682,16 +682,16 @@ def test_multithreaded_oracle_matches_sequential(oracle):
         try:
             pxd, conic, aabb, _ = oracle.gauss_project(s["xyz"], s["quat"], s["scale"], m, shape)
             t2i, i2p, keys = oracle.tile2pnt_gauss(pxd, aabb, shape, ts, with_keys=True)
-            img, alpha, nwalk, tlast = oracle.gauss_rasterize(shape, pxd, conic, aabb,
+            img, alpha, nwalk, tfinal = oracle.gauss_rasterize(shape, pxd, conic, aabb,
                                                               s["opacity"], s["rgb"], ts, t2i, i2p)
             d_img = (2 * img - 1).astype(np.float32)
             fn = oracle.gauss_rasterize_bwd if threads == 1 else oracle.gauss_rasterize_bwd_mt
-            d = fn(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p, nwalk, tlast,
+            d = fn(shape, pxd, conic, aabb, s["opacity"], s["rgb"], ts, t2i, i2p, nwalk, tfinal,
                    d_img)
             g = oracle.gauss_project_bwd(s["xyz"], s["quat"], s["scale"], m, shape, d[0], d[1])
         finally:
             oracle.set_threads(1)
-        out = [pxd, conic, aabb, t2i, i2p, keys, img, alpha, nwalk, tlast] + list(d) + list(g)
+        out = [pxd, conic, aabb, t2i, i2p, keys, img, alpha, nwalk, tfinal] + list(d) + list(g)
         if threads == 1:
             ref = out
             continue
