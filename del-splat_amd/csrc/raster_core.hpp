@@ -322,7 +322,7 @@ __device__ __forceinline__ bool box_take(int g, const bool (&walking)[GroupMap<T
 // index iy * img_w + ix); alpha / nwalk / tfinal may be null. The entry loop is
 // outermost so one LDS read of a record serves all PPT pixels of a thread.
 // kState: track the adjoint's state (nwalk, tfinal) and the tile cost; a
-// forward asked for neither leaves their two selects per pixel and entry out.
+// forward asked for neither leaves the nwalk select per pixel and entry out.
 template <int TS, int NT, typename Loader, bool kState = true>
 __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32_t img_w,
                                                 uint32_t img_h, uint32_t start, uint32_t end,
@@ -597,7 +597,9 @@ struct GradSink {
 
 // LDS of the adjoint, declared once per kernel by raster_tile_bwd (each
 // template instance declaring its own arrays would double the footprint).
-// Batches hold B <= 128 list entries. With one or two waves, acc holds one
+// Batches hold B = 64 list entries: 14.8 KB in all at 32x32 tiles and 256
+// threads, so registers rather than LDS set the occupancy there (7 waves per
+// SIMD; batches of 128 took 29.7 KB and held it to 5). With one or two waves, acc holds one
 // partial-sum slot per wave (waves combine by plain stores); with more, one
 // slot that the waves add into with LDS float atomics (per-wave slots would
 // take so much LDS that fewer workgroups fit on a CU).
