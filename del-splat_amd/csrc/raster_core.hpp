@@ -599,10 +599,11 @@ struct GradSink {
 // template instance declaring its own arrays would double the footprint).
 // Batches hold B = 64 list entries: 14.8 KB in all at 32x32 tiles and 256
 // threads, so registers rather than LDS set the occupancy there (7 waves per
-// SIMD; batches of 128 took 29.7 KB and held it to 5). With one or two waves, acc holds one
-// partial-sum slot per wave (waves combine by plain stores); with more, one
-// slot that the waves add into with LDS float atomics (per-wave slots would
-// take so much LDS that fewer workgroups fit on a CU).
+// SIMD; batches of 128 took 29.7 KB and held it to 5). With one or two
+// waves, acc holds one partial-sum slot per wave (waves combine by plain
+// stores); with more, one slot that the waves add into with LDS float
+// atomics (per-wave slots would take so much LDS that fewer workgroups fit on
+// a CU).
 template <int TS, int NT>
 struct BwdLds {
   static constexpr int B = 64;
