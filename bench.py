@@ -707,8 +707,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not run", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding across "
+              "GPUs (views per GPU, one all-reduce of the gradients) is not implemented",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
