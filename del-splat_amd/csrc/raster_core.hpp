@@ -446,7 +446,10 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cg[k] = fmaf(q1.w, eo, cg[k]);
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
                 if constexpr (kState) walked[k] = in ? wpos : walked[k];
-                tr[k] = tr[k] * (1.f - e);
+                // 1 - e floored at 2^-24 (only a splat with e within 2^-24
+                // of 1, or above it, is affected, and it ends the walk): T
+                // stays a power-of-two multiple of the adjoint's T, never 0
+                tr[k] = tr[k] * fmaxf(1.f - e, 0x1p-24f);
                 asum[k] = asum[k] + eo;
               }
             }
@@ -780,7 +783,9 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           const float gk = GM::kRowShared ? splat_weight_cyy(in, ke, dx, dy, cyy)
                                           : splat_weight(in, ke, dx, dy);
           const float e = gk * q1.y;
-          const float ome = 1.f - e;
+          // 1 - e floored as in the forward: a last splat with e = 1 would
+          // otherwise give 0 * inf (tr from tfinal, and gb * inv)
+          const float ome = fmaxf(1.f - e, 0x1p-24f);
           const float inv = __builtin_amdgcn_rcpf(ome);
           tr[k] = tr[k] * inv;
           const float t = tr[k];

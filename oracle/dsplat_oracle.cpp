@@ -437,7 +437,10 @@ inline PixelOut<T> raster_pixel(uint32_t ix, uint32_t iy, const T* pixxydepth, c
     o.c[1] += rgb[3 * p + 1] * eo;
     o.c[2] += rgb[3 * p + 2] * eo;
     o.walked = it + 1;
-    alpha_occu *= T(1) - e;
+    // 1 - e floored at 2^-24: only a splat with e within 2^-24 of 1 (or
+    // above) is affected, the walk ends there, and the colour and alpha do
+    // not depend on it; the stored transmittance then divides back exactly
+    alpha_occu *= std::max(T(1) - e, T(0x1p-24));
     o.tfinal = alpha_occu;
     o.alpha_sum += eo;
     // the reference compares the float sum against the double 0.999
@@ -551,8 +554,8 @@ void gauss_rasterize_bwd(uint32_t img_w, uint32_t img_h, const T* pixxydepth, co
         const T gk = std::exp(T(-0.5) * q);
         const T op = opacity[p];
         const T e = gk * op;
-        trans = trans / (T(1) - e);  // now in front of it
-        const T one_m_e = T(1) - e;
+        const T one_m_e = std::max(T(1) - e, T(0x1p-24));  // floored as in the forward
+        trans = trans / one_m_e;  // now in front of it
         // dL/de
         T dl_de = 0;
         for (int k = 0; k < 3; ++k) {

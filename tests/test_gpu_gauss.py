@@ -715,3 +715,41 @@ def test_box_edges_on_pixel_centres(oracle, ts, threads):
         wide[:, e] = np.nextafter(wide[:, e], np.float32(d))
     img2 = SG.rasterize_fwd(shape, ts, _t(wide), g_t2i, g_i2p, threads)[0]
     assert (img2 != img).any()
+
+
+@pytest.mark.parametrize("ts,threads", [(16, 64), (16, 256), (32, 256), (32, 512)])
+def test_opaque_splat_on_a_pixel_centre(oracle, ts, threads):
+    """An opacity-1 splat whose mean is a pixel centre (e = 1 there, the walk
+    ends on it): the stored transmittance is floored, never 0, and the
+    adjoint's gradients are finite and match the oracle's."""
+    import del_splat_dlpack.SplatGauss.torch as SG
+    shape = (32, 32)
+    pxd = np.array([[8.5, 8.5, 0.5], [8.5, 8.5, 0.2], [20.3, 11.7, 0.4]], np.float32)
+    conic = np.array([[0.1, 0.0, 0.1], [0.1, 0.0, 0.1], [0.05, 0.01, 0.08]], np.float32)
+    box = np.array([[0, 0, 32, 32]] * 3, np.float32)
+    op = np.array([1.0, 0.5, 1.0], np.float32)
+    rgb = np.array([[0.2, 0.3, 0.4], [0, 1, 0], [0.9, 0.1, 0.5]], np.float32)
+    splat2 = np.zeros((3, 16), np.float32)
+    splat2[:, 0:2] = pxd[:, :2]
+    splat2[:, 2:5] = conic
+    splat2[:, 5] = op
+    splat2[:, 6:9] = rgb
+    splat2[:, 9:13] = box
+    t2i, i2p = oracle.tile2pnt_gauss(pxd, box, shape, ts)
+    o_img, o_alpha, o_nwalk, o_tfinal = oracle.gauss_rasterize(shape, pxd, conic, box, op, rgb,
+                                                              ts, t2i, i2p)
+    g_t2i, g_i2p = _t(t2i.view(np.int32)), _t(i2p.view(np.int32))
+    img, alpha, nwalk, tfinal = SG.rasterize_fwd(shape, ts, _t(splat2), g_t2i, g_i2p, threads)
+    np.testing.assert_allclose(img.cpu().numpy(), o_img, atol=1e-5)
+    np.testing.assert_array_equal(nwalk.cpu().numpy().view(np.uint32), o_nwalk)
+    assert tfinal.cpu().numpy()[8, 8] == np.float32(2.0 ** -24)
+    d_img = np.ones((shape[1], shape[0], 3), np.float32)
+    g = SG.rasterize_bwd(shape, ts, _t(splat2), g_t2i, g_i2p, nwalk, tfinal, _t(d_img), None,
+                         threads)
+    o = oracle.gauss_rasterize_bwd(shape, pxd, conic, box, op, rgb, ts, t2i, i2p,
+                                   nwalk.cpu().numpy().view(np.uint32), tfinal.cpu().numpy(),
+                                   d_img)
+    for name, a, b in zip(("d_pixxy", "d_conic", "d_opacity", "d_rgb"), g, o):
+        a = a.cpu().numpy()
+        assert np.isfinite(a).all(), name
+        check_grad(a, b, f"ts={ts} threads={threads} {name}")

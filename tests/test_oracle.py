@@ -699,3 +699,30 @@ def test_multithreaded_oracle_matches_sequential(oracle):
             np.testing.assert_array_equal(a, b)
         for a, b in zip(out[10:], ref[10:]):
             check_grad(a, b, "multithreaded adjoint", rel=1e-4, abs_frac=1e-6)
+
+
+def test_opaque_splat_on_a_pixel_centre_gives_finite_gradients(oracle):
+    """An opacity-1 splat whose mean is a pixel centre has e = 1 there: the
+    walk ends on it, the stored transmittance is floored at 2^-24 of the one
+    in front of it (never 0) and the adjoint divides it back exactly, so no
+    gradient is 0 * inf. A second splat behind it never contributes there."""
+    shape = (16, 16)
+    pxd = np.array([[8.5, 8.5, 0.5], [8.5, 8.5, 0.2]], np.float32)  # first is nearer
+    conic = np.array([[0.1, 0.0, 0.1]] * 2, np.float32)
+    aabb = np.array([[0, 0, 16, 16]] * 2, np.float32)
+    op = np.array([1.0, 0.5], np.float32)
+    rgb = np.array([[0.2, 0.3, 0.4], [0, 1, 0]], np.float32)
+    t2i, i2p = oracle.tile2pnt_gauss(pxd, aabb, shape, 16)
+    img, alpha, nwalk, tfinal = oracle.gauss_rasterize(shape, pxd, conic, aabb, op, rgb, 16, t2i,
+                                                       i2p)
+    np.testing.assert_array_equal(img[8, 8], rgb[0])  # e = 1: only the first splat
+    assert alpha[8, 8] == 1.0 and nwalk[8, 8] == 1
+    assert tfinal[8, 8] == np.float32(2.0 ** -24)
+    d_img = np.ones_like(img)
+    for dt in (np.float32, np.float64):
+        g = oracle.gauss_rasterize_bwd(shape, pxd, conic, aabb, op, rgb, 16, t2i, i2p, nwalk,
+                                       tfinal, d_img, dtype=dt)
+        for a in g:
+            assert np.isfinite(a).all()
+        # at the opaque splat's centre pixel dL/drgb = T e = 1 per channel
+        assert (g[3][:3] >= 1.0).all()
