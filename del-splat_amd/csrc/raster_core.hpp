@@ -73,6 +73,11 @@ struct ArrayLoader {
 // 2^(q * -log2(e)/2), a few ulps from expf(-q/2). Shared by the forward and
 // the adjoint, so every workgroup shape produces bit-identical pixels and the
 // adjoint recomputes exactly the forward's weights.
+// Opacity as the walks stage it: at most 1 - 2^-24, the largest float below
+// 1 (an opacity of 1 becomes 1 - 2^-24, a relative change of 2^-24; the
+// model's sigmoid opacities lie below 1 anyway). Gradients pass through.
+__device__ __forceinline__ float stage_opacity(float o) { return fminf(o, 0x1.fffffep-1f); }
+
 struct ConicQ {
   float a, b2, c;  // a, 2b, c
 };
@@ -82,21 +87,25 @@ __device__ __forceinline__ ConicQ conic_q(float a, float b, float c) {
 }
 
 // exp(-q/2) where in, else 0 (2^-inf = 0: a select on the argument keeps
-// the exp unpredicated). cyy = (c dy) dy, the last term of q, passed in by
+// the exp unpredicated), clamped to [0, 1] by the exp's own output clamp (no
+// extra instruction): q >= 0 for a positive-definite conic, and the clamp
+// only removes rounding's 1 + ulp where q cancels to just below 0. With the
+// opacity staged at most 1 - 2^-24 this bounds e = g * opacity by 1 - 2^-24
+// (rounding is monotone), so 1 - e >= 2^-24 holds for every splat. cyy = (c dy) dy, the last term of q, passed in by
 // callers that share it between pixels of one row (the same roundings).
 __device__ __forceinline__ float splat_weight_cyy(bool in, const ConicQ& k, float dx, float dy,
                                                   float cyy) {
 #pragma clang fp contract(off)
   constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
   const float q = ((k.a * dx) * dx + k.b2 * (dx * dy)) + cyy;
-  return __builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff());
+  return __builtin_amdgcn_fmed3f(__builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff()), 0.f, 1.f);
 }
 
 __device__ __forceinline__ float splat_weight(bool in, const ConicQ& k, float dx, float dy) {
 #pragma clang fp contract(off)
   constexpr float kS = -0.72134752044448170368f;  // -log2(e) / 2
   const float q = ((k.a * dx) * dx + k.b2 * (dx * dy)) + (k.c * dy) * dy;
-  return __builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff());
+  return __builtin_amdgcn_fmed3f(__builtin_amdgcn_exp2f(in ? q * kS : -__builtin_inff()), 0.f, 1.f);
 }
 
 // Pixel -> thread mapping of a tile (forward and adjoint). The TS x TS tile
@@ -373,7 +382,7 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
       // the conic with b doubled once per entry (every wave uses it)
       const ConicQ ke = conic_q(q0.z, q0.w, q1.x);
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
-      s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
+      s1[threadIdx.x] = make_float4(ke.c, stage_opacity(q1.y), q1.z, q1.w);
       s2[threadIdx.x] = q2;
       // the record's free second slot carries the entry's walk position + 1
       // (what a pixel it reaches records as nwalk): read from LDS it is a
@@ -446,10 +455,9 @@ __device__ __forceinline__ void raster_tile_fwd(uint32_t tx, uint32_t ty, uint32
                 cg[k] = fmaf(q1.w, eo, cg[k]);
                 cbl[k] = fmaf(q2.x, eo, cbl[k]);
                 if constexpr (kState) walked[k] = in ? wpos : walked[k];
-                // 1 - e floored at 2^-24 (only a splat with e within 2^-24
-                // of 1, or above it, is affected, and it ends the walk): T
-                // stays a power-of-two multiple of the adjoint's T, never 0
-                tr[k] = tr[k] * fmaxf(1.f - e, 0x1p-24f);
+                // 1 - e >= 2^-24 by the staged bounds on g and opacity: T
+                // is never 0, and the adjoint divides it back out
+                tr[k] = tr[k] * (1.f - e);
                 asum[k] = asum[k] + eo;
               }
             }
@@ -709,7 +717,7 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
       // the record's free slots for the flush
       const ConicQ ke = conic_q(q0.z, q0.w, q1.x);
       s0[threadIdx.x] = make_float4(q0.x, q0.y, ke.a, ke.b2);
-      s1[threadIdx.x] = make_float4(ke.c, q1.y, q1.z, q1.w);
+      s1[threadIdx.x] = make_float4(ke.c, stage_opacity(q1.y), q1.z, q1.w);
       s2[threadIdx.x] = q2;
       s3[threadIdx.x] = make_float4(q3.x, q0.z, q0.w, q1.x);
       spid[threadIdx.x] = p;
@@ -783,10 +791,9 @@ __device__ __forceinline__ void raster_tile_bwd_impl(uint32_t tx, uint32_t ty, u
           const float gk = GM::kRowShared ? splat_weight_cyy(in, ke, dx, dy, cyy)
                                           : splat_weight(in, ke, dx, dy);
           const float e = gk * q1.y;
-          // 1 - e floored as in the forward: a last splat with e = 1 would
-          // otherwise give 0 * inf (tr from tfinal, and gb * inv)
-          const float ome = fmaxf(1.f - e, 0x1p-24f);
-          const float inv = __builtin_amdgcn_rcpf(ome);
+          // 1 - e >= 2^-24 as in the forward: an opaque splat on a pixel
+          // centre gives a large finite 1 / (1 - e), never 0 * inf
+          const float inv = __builtin_amdgcn_rcpf(1.f - e);
           tr[k] = tr[k] * inv;
           const float t = tr[k];
           // dL/de = g.(c T - B/(1-e)) + ga (T - A/(1-e)) = T gc - gb/(1-e)

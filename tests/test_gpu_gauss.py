@@ -720,8 +720,10 @@ def test_box_edges_on_pixel_centres(oracle, ts, threads):
 @pytest.mark.parametrize("ts,threads", [(16, 64), (16, 256), (32, 256), (32, 512)])
 def test_opaque_splat_on_a_pixel_centre(oracle, ts, threads):
     """An opacity-1 splat whose mean is a pixel centre (e = 1 there, the walk
-    ends on it): the stored transmittance is floored, never 0, and the
-    adjoint's gradients are finite and match the oracle's."""
+    ends on it): the stored transmittance is 2^-24 of the one in front, never
+    0 (the kernels stage opacity at most 1 - 2^-24, the oracle floors 1 - e at
+    2^-24, which agree here), and the adjoint's gradients are finite and
+    match the oracle's."""
     import del_splat_dlpack.SplatGauss.torch as SG
     shape = (32, 32)
     pxd = np.array([[8.5, 8.5, 0.5], [8.5, 8.5, 0.2], [20.3, 11.7, 0.4]], np.float32)
