@@ -701,6 +701,32 @@ def test_multithreaded_oracle_matches_sequential(oracle):
             check_grad(a, b, "multithreaded adjoint", rel=1e-4, abs_frac=1e-6)
 
 
+def test_staged_opacity_bound_keeps_one_minus_e_above_2_pow_minus_24():
+    """The HIP walks need no per-pixel floor on 1 - e: they clamp the splat
+    weight g to [0, 1] and stage the opacity at most 1 - 2^-24 (the largest
+    float32 below 1). float32 products round to nearest, monotonically, so
+    e = g * o never exceeds 1 - 2^-24 and 1 - e >= 2^-24, exact for
+    e >= 1/2. Checked for every float32 g in [1 - 2^-12, 1] (where e comes
+    closest to 1) and a spread of smaller ones, against the float64 value:
+    the subtraction is exact, so 1 - e matches the float64 difference."""
+    f32 = np.float32
+    o = f32(1.0) - f32(2.0 ** -24)
+    assert o == np.nextafter(f32(1.0), f32(0.0))
+    lo = np.float32(1.0 - 2.0 ** -12).view(np.uint32)
+    hi = np.float32(1.0).view(np.uint32)
+    g = np.arange(lo, hi + 1, dtype=np.uint32).view(np.float32)
+    g = np.concatenate([g, np.linspace(0, 1, 100001, dtype=np.float32), [f32(0.5), f32(0.75)]])
+    e = g * o
+    assert e.dtype == np.float32
+    assert (e <= o).all()
+    ome = f32(1.0) - e
+    assert (ome >= f32(2.0 ** -24)).all()
+    big = e >= f32(0.5)
+    np.testing.assert_array_equal(ome[big].astype(np.float64), 1.0 - e[big].astype(np.float64))
+    # the closest case: g = 1 gives 1 - e = 2^-24 exactly, the oracle's floor
+    assert f32(1.0) - f32(1.0) * o == f32(2.0 ** -24)
+
+
 def test_opaque_splat_on_a_pixel_centre_gives_finite_gradients(oracle):
     """An opacity-1 splat whose mean is a pixel centre has e = 1 there: the
     walk ends on it, the stored transmittance is floored at 2^-24 of the one
